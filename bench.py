@@ -1,0 +1,144 @@
+"""Benchmark: second-order FP64 residual throughput (Mcell-residuals/s) on one MI355X.
+
+Workload: inviscid transonic airfoil operator (HLLC, weighted least squares, Venkatakrishnan
+K=5, M0.8, alpha 1.25 deg) on the refined 2048x512 NACA 4-digit (t=0.12) O-mesh with the outer
+256 rings split into triangles: 1,572,864 cells, RCM-ordered. A step is one full
+FlowFV::compute_residual (gradients + limiter pass, face-flux / gather pass with local time
+steps) over a synthetic, seeded perturbed free-stream state resident in HBM.
+
+Prints one JSON line. The CPU baseline is this repository's CPU oracle in its
+reference-structured OpenMP mode (face loops with atomic scatters), timed on a bounded sample
+of the same workload on the host cores.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# bytes per cell of the face-flux pass, from the two-pass compulsory-traffic model of the
+# residual (408 B/cell for this operator): u read 32 + limited gradient read 64 + residual
+# write 32 + time-step write 8 + area 8 + face data 48 B/face x 1.668 faces/cell = 224 B/cell
+FLUX_PASS_BYTES_PER_CELL = 224.0
+RESIDUAL_BYTES_PER_CELL = 408.0
+HBM_PEAK_GBS = 8000.0
+
+
+def parse():
+    p = argparse.ArgumentParser()
+    p.add_argument("--gpus", type=int, default=1)
+    p.add_argument("--steps", type=int, default=1000)
+    p.add_argument("--warmup", type=int, default=20)
+    p.add_argument("--cpu-seconds", type=float, default=12.0,
+                   help="bounded CPU-baseline sample (seconds of CPU work)")
+    p.add_argument("--no-cpu-baseline", action="store_true")
+    return p.parse_args()
+
+
+def build_case():
+    import numpy as np
+    from fvens_amd import fvx, meshgen, states
+    raw = meshgen.mesh_k2l()
+    raw = raw.permuted(fvx.rcm_permutation(raw))
+    bcs = [(2, "slipwall"), (4, "farfield")]
+    pcfg = fvx.physics_cfg(Minf=0.8, aoa_deg=1.25, bcs=bcs)
+    ncfg = fvx.numerics_cfg(flux="HLLC", gradient="LEASTSQUARES", recon="VENKATAKRISHNAN",
+                            limiter_param=5.0)
+    u = states.perturbed_freestream(raw, Minf=0.8, aoa_deg=1.25, L=1.0)
+    return raw, bcs, pcfg, ncfg, np.ascontiguousarray(u)
+
+
+def cpu_baseline(raw, bcs, u, budget_s):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    m = po.Mesh(raw)
+    d = po.Disc(m, po.make_config(Minf=0.8, aoa_deg=1.25, flux="HLLC", gradient="LEASTSQUARES",
+                                  recon="VENKATAKRISHNAN", limiter_param=5.0, bcs=bcs))
+    d.residual(u, parallel=1)                       # warm-up
+    n, t = 0, 0.0
+    while t < budget_s and n < 50:
+        t0 = time.perf_counter()
+        d.residual(u, parallel=1)
+        t += time.perf_counter() - t0
+        n += 1
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return dict(value=raw.nelem * n / t / 1e6, unit="Mcell-residuals/s", cores=threads,
+                kind="port",
+                sample=f"{n} full residual evaluations of the same 1,572,864-cell operator and "
+                       f"state ({t:.1f} s), oracle OpenMP face loops with atomic scatters")
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported "
+              "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from fvens_amd import fvx
+    assert torch.cuda.is_available(), "no GPU"
+    raw, bcs, pcfg, ncfg, u_host = build_case()
+    mesh = fvx.Mesh(raw)
+    disc = fvx.FlowFV(mesh, pcfg, ncfg)
+    u = torch.tensor(u_host, device="cuda")
+    r = torch.zeros_like(u)
+    dt = torch.zeros(raw.nelem, dtype=torch.float64, device="cuda")
+    for _ in range(a.warmup):
+        disc.compute_residual(u, r, True, dt)
+    torch.cuda.synchronize()
+    disc.profile_enable(True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        disc.compute_residual(u, r, True, dt)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    flux_ms, flux_n = disc.profile_read("flux")
+    grad_ms, grad_n = disc.profile_read("gradients")
+    disc.profile_enable(False)
+    assert torch.isfinite(r).all().item() and torch.isfinite(dt).all().item()
+    ms_per_step = (t1 - t0) * 1e3 / a.steps
+    value = raw.nelem * a.steps / (t1 - t0) / 1e6
+    flux_avg_s = flux_ms / flux_n / 1e3
+    grad_avg_s = grad_ms / grad_n / 1e3
+    achieved = FLUX_PASS_BYTES_PER_CELL * raw.nelem / flux_avg_s / 1e9
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "k_flux_traffic.json")
+    if os.path.exists(tf):
+        with open(tf) as fh:
+            traffic = json.load(fh).get("hbm_bytes_per_launch")
+    out = {
+        "metric": "Mcell-residuals/s; time to 1e-10 residual: warm-start+FP32 PC vs plain FP64",
+        "value": round(value, 3),
+        "unit": "Mcell-residuals/s",
+        "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic",
+        "config": {"workload": "inviscid transonic airfoil residual, NACA 4-digit t=0.12 O-mesh "
+                               "2048x512 (outer 256 rings split), RCM order",
+                   "cells": raw.nelem, "faces": int(mesh.naface), "flux": "HLLC",
+                   "gradient": "weighted least squares", "limiter": "Venkatakrishnan K=5",
+                   "mach": 0.8, "aoa_deg": 1.25, "state": "seeded perturbed free stream",
+                   "parallelism": "single GPU"},
+        "kernels_ms": {"k_gradients": round(grad_avg_s * 1e3, 5),
+                       "k_flux": round(flux_avg_s * 1e3, 5)},
+        "roofline": {"bound": "hbm", "kernel": "k_flux", "achieved": round(achieved, 1),
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes_per_cell": FLUX_PASS_BYTES_PER_CELL},
+        "residual_roofline_frac": round(RESIDUAL_BYTES_PER_CELL * raw.nelem
+                                        / ((flux_avg_s + grad_avg_s) * 1e9) / HBM_PEAK_GBS, 4),
+    }
+    if not a.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(raw, bcs, u_host, a.cpu_seconds)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,172 @@
+/* api.cpp -- the extern "C" boundary of libfvx (declared in include/fvx.h). Converts internal
+ * exceptions to status codes and keeps the last error text per thread. */
+#include "fvx_internal.hpp"
+#include <cstring>
+#include <memory>
+
+namespace {
+thread_local std::string g_last_error;
+
+template <class F> fvx_status guarded(F&& f)
+{
+	try { f(); return FVX_OK; }
+	catch(const fvx::Error& e) { g_last_error = e.what(); return e.code; }
+	catch(const std::bad_alloc&) { g_last_error = "host allocation failed"; return FVX_ERR_ALLOC; }
+	catch(const std::exception& e) { g_last_error = e.what(); return FVX_ERR_INVALID; }
+	catch(...) { g_last_error = "unknown error"; return FVX_ERR_INVALID; }
+}
+
+void require(bool c, const char* msg)
+{
+	if(!c) throw fvx::Error(FVX_ERR_INVALID, msg);
+}
+}
+
+extern "C" {
+
+const char* fvx_last_error(void) { return g_last_error.c_str(); }
+
+fvx_status fvx_mesh_rcm(const fvx_mesh_desc* desc, int* perm)
+{
+	return guarded([&] {
+		require(desc && perm, "null argument");
+		fvx::rcm_order(*desc, perm);
+	});
+}
+
+fvx_status fvx_mesh_create(const fvx_mesh_desc* desc, fvx_mesh* out)
+{
+	return guarded([&] {
+		require(desc && out, "null argument");
+		std::unique_ptr<fvx_mesh_s> m(new fvx_mesh_s);
+		fvx::build_mesh(*m, *desc);
+		*out = m.release();
+	});
+}
+
+fvx_status fvx_mesh_sizes(fvx_mesh m, int* s)
+{
+	return guarded([&] {
+		require(m && s, "null argument");
+		s[0] = m->npoin; s[1] = m->nelem; s[2] = m->nbface; s[3] = m->ninface; s[4] = m->naface;
+	});
+}
+
+fvx_status fvx_mesh_faces(fvx_mesh m, int* intfac, double* facemetric, double* area,
+                          double* centres)
+{
+	return guarded([&] {
+		require(m != nullptr, "null mesh");
+		if(intfac) std::memcpy(intfac, m->intfac.data(), sizeof(int)*m->intfac.size());
+		if(facemetric) std::memcpy(facemetric, m->facemetric.data(), sizeof(double)*m->facemetric.size());
+		if(area) std::memcpy(area, m->area.data(), sizeof(double)*m->area.size());
+		if(centres) std::memcpy(centres, m->rc.data(), sizeof(double)*m->rc.size());
+	});
+}
+
+fvx_status fvx_mesh_destroy(fvx_mesh m)
+{
+	return guarded([&] { delete m; });
+}
+
+fvx_status fvx_disc_create(fvx_mesh m, const fvx_physics_cfg* p, const fvx_numerics_cfg* n,
+                           fvx_disc* out)
+{
+	return guarded([&] {
+		require(m && p && n && out, "null argument");
+		std::unique_ptr<fvx_disc_s> d(new fvx_disc_s);
+		d->m = m;
+		d->pc = *p;
+		d->nc = *n;
+		fvx::build_disc(*d);
+		*out = d.release();
+	});
+}
+
+fvx_status fvx_disc_destroy(fvx_disc d)
+{
+	return guarded([&] { delete d; });
+}
+
+fvx_status fvx_freestream_state(fvx_disc d, double* out)
+{
+	return guarded([&] {
+		require(d && out, "null argument");
+		for(int k = 0; k < 4; k++) out[k] = d->gas.uinf[k];
+	});
+}
+
+fvx_status fvx_compute_residual(fvx_disc d, const double* u, double* r, int gettimesteps,
+                                double* dt, void* stream)
+{
+	return guarded([&] {
+		require(d && u && r, "null argument");
+		require(!gettimesteps || dt, "time steps requested without an output array");
+		fvx::launch_residual(*d, u, r, gettimesteps, dt, static_cast<hipStream_t>(stream));
+	});
+}
+
+fvx_status fvx_get_gradients(fvx_disc d, const double* u, double* grads, void* stream)
+{
+	return guarded([&] {
+		require(d && u && grads, "null argument");
+		fvx::launch_gradients(*d, u, grads, nullptr, 1, static_cast<hipStream_t>(stream));
+	});
+}
+
+fvx_status fvx_get_primitive_gradients(fvx_disc d, const double* u, double* grads,
+                                       void* stream)
+{
+	return guarded([&] {
+		require(d && u && grads, "null argument");
+		const int rec = d->nc.reconstruction;
+		const bool limited = rec == FVX_RECON_VENKATAKRISHNAN || rec == FVX_RECON_BARTHJESPERSEN;
+		fvx::launch_gradients(*d, u, grads, limited ? d->d_rgrad.p : nullptr, 0,
+		                      static_cast<hipStream_t>(stream));
+	});
+}
+
+fvx_status fvx_profile_enable(fvx_disc d, int on)
+{
+	return guarded([&] {
+		require(d != nullptr, "null handle");
+		if(on) { d->prof.collect(); d->prof.reset(); }
+		d->prof.on = on != 0;
+	});
+}
+
+fvx_status fvx_profile_read(fvx_disc d, int kernel, double* total_ms, long* count)
+{
+	return guarded([&] {
+		require(d && total_ms && count, "null argument");
+		require(kernel >= 0 && kernel < fvx::Profiler::NKERNEL, "kernel id out of range");
+		d->prof.collect();
+		*total_ms = d->prof.total_ms[kernel];
+		*count = d->prof.count[kernel];
+	});
+}
+
+fvx_status fvx_functionals(fvx_disc d, const double* u, int wall_tag, double* out, void* stream)
+{
+	return guarded([&] {
+		require(d && u && out, "null argument");
+		hipStream_t s = static_cast<hipStream_t>(stream);
+		fvx::launch_gradients(*d, u, d->d_grad.p, nullptr, 1, s);
+		const int nb = fvx::functional_blocks();
+		fvx::launch_functionals(*d, u, d->d_grad.p, wall_tag, d->d_scratch.p, s);
+		std::vector<double> part((size_t)nb*5);
+		FVX_HIP(hipMemcpyAsync(part.data(), d->d_scratch.p, sizeof(double)*part.size(),
+		                       hipMemcpyDeviceToHost, s));
+		FVX_HIP(hipStreamSynchronize(s));
+		double acc[5] = {0, 0, 0, 0, 0};
+		for(int b = 0; b < nb; b++)
+			for(int k = 0; k < 5; k++) acc[k] += part[(size_t)b*5+k];
+		require(acc[3] > 0, "no boundary face carries the requested wall marker");
+		out[0] = acc[0]/acc[3];
+		out[1] = acc[1]/acc[3];
+		out[2] = acc[2]/acc[3];
+		out[3] = std::sqrt(acc[4]);
+	});
+}
+
+}

@@ -1,0 +1,37 @@
+/* flux_inst.hip -- instantiates the flux kernel for one numerical flux (FVX_FLUX_ID is set
+ * by the build, one object per flux so the instantiations compile in parallel) and exposes a
+ * launcher selecting the reconstruction mode and the viscous variant. */
+#include "flux_kernel.hpp"
+
+#ifndef FVX_FLUX_ID
+#error "FVX_FLUX_ID must be defined"
+#endif
+
+#define FVX_CAT2(a, b) a##b
+#define FVX_CAT(a, b) FVX_CAT2(a, b)
+
+namespace fvx {
+
+void FVX_CAT(launch_flux_, FVX_FLUX_ID)(int mode, bool visc, int ncell, const double* u,
+                                        const double* rgrad, const double* grad,
+                                        const int* nbr, const int* face, const FaceGeom* fg,
+                                        const double* centre, const double* area,
+                                        const int* bcid, const GasParams& gp, const BCTable& bt,
+                                        double* res, double* dt, int getdt, hipStream_t s)
+{
+	const int nblk = (ncell + CELLS_PER_BLOCK - 1)/CELLS_PER_BLOCK;
+#define FVX_LAUNCH(M, V) hipLaunchKernelGGL((k_flux<FVX_FLUX_ID, M, V>), dim3(nblk), dim3(BLOCK), \
+		0, s, ncell, u, rgrad, grad, nbr, face, fg, centre, area, bcid, gp, bt, res, dt, getdt)
+	if(!visc) {
+		if(mode == 0) FVX_LAUNCH(0, false);
+		else if(mode == 1) FVX_LAUNCH(1, false);
+		else FVX_LAUNCH(2, false);
+	} else {
+		if(mode == 0) FVX_LAUNCH(0, true);
+		else if(mode == 1) FVX_LAUNCH(1, true);
+		else FVX_LAUNCH(2, true);
+	}
+#undef FVX_LAUNCH
+}
+
+}

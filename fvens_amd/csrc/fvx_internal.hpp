@@ -1,0 +1,131 @@
+/* fvx_internal.hpp -- internal types shared by the host code and kernel launchers of libfvx. */
+#ifndef FVX_INTERNAL_HPP
+#define FVX_INTERNAL_HPP
+
+#include <hip/hip_runtime.h>
+#include <string>
+#include <vector>
+#include <stdexcept>
+#include "../../include/fvx.h"
+#include "gas.hpp"
+
+namespace fvx {
+
+/// Error carrying an fvx_status across internal C++ code; converted at the C boundary.
+struct Error : std::runtime_error {
+	int code;
+	Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define FVX_HIP(call) do { hipError_t e_ = (call); if(e_ != hipSuccess) \
+	throw ::fvx::Error(FVX_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); } while(0)
+
+/// Marker in a cell's neighbour slot that is not used (triangles have 3 faces).
+constexpr int SLOT_UNUSED = -0x40000000;
+
+/// Face geometry record: unit normal pointing from the left (lower) cell to the right cell,
+/// length and midpoint (48 bytes, three 16-byte loads).
+struct alignas(16) FaceGeom {
+	double nx, ny, len, mx, my, pad;
+};
+
+/// Device buffer owned by a handle.
+template <class T> struct DBuf {
+	T* p = nullptr;
+	size_t n = 0;
+	void alloc(size_t count) {
+		release();
+		if(count == 0) return;
+		if(hipMalloc((void**)&p, count*sizeof(T)) != hipSuccess) {
+			p = nullptr;
+			throw Error(FVX_ERR_ALLOC, "hipMalloc failed");
+		}
+		n = count;
+	}
+	void upload(const std::vector<T>& h) {
+		alloc(h.size());
+		if(n) FVX_HIP(hipMemcpy(p, h.data(), n*sizeof(T), hipMemcpyHostToDevice));
+	}
+	void release() { if(p) (void)hipFree(p); p = nullptr; n = 0; }
+	~DBuf() { release(); }
+	DBuf() = default;
+	DBuf(const DBuf&) = delete;
+	DBuf& operator=(const DBuf&) = delete;
+};
+
+}
+
+/// Host + device mesh. Cells are in the caller's order; faces follow the reference numbering
+/// (boundary faces in input order, then interior faces in element order with L < R).
+struct fvx_mesh_s {
+	int npoin = 0, nelem = 0, nbface = 0, ninface = 0, naface = 0;
+	// host topology and geometry
+	std::vector<double> coords;
+	std::vector<int> inpoel, nnode, bface, btag;
+	std::vector<int> esuel, elemface;      // [nelem][4]
+	std::vector<int> intfac;               // [naface][4]
+	std::vector<double> facemetric;        // [naface][3]
+	std::vector<double> area, rc;          // [nelem], [nelem][2]
+	std::vector<double> gr, rcbp;          // face midpoints [naface][2], ghost centres [nbface][2]
+	std::vector<double> hlong;             // longest edge per cell
+	// device copies
+	fvx::DBuf<int> d_nbr, d_face;          // [nelem][4] slots: neighbour / face index
+	fvx::DBuf<fvx::FaceGeom> d_fgeom;      // [naface]
+	fvx::DBuf<double> d_centre;            // [nelem][2]
+	fvx::DBuf<double> d_area;              // [nelem]
+	fvx::DBuf<int> d_bhost, d_btag;        // [nbface] host cell and marker of boundary faces
+};
+
+namespace fvx {
+/// Optional per-kernel timing with HIP events recorded on the launch stream around each
+/// kernel; read back (and the stream synchronised) only on request.
+struct Profiler {
+	static constexpr int NKERNEL = 8;
+	bool on = false;
+	std::vector<hipEvent_t> pool;          // start/stop pairs
+	std::vector<int> kid;                  // kernel id of each pair
+	size_t used = 0;
+	double total_ms[NKERNEL] = {};
+	long count[NKERNEL] = {};
+	void start(int k, hipStream_t s);
+	void stop(hipStream_t s);
+	void collect();
+	void reset();
+	~Profiler();
+};
+enum { PROF_GRADIENTS = 0, PROF_FLUX = 1, PROF_JACOBIAN = 2, PROF_PC = 3 };
+}
+
+/// Discretisation handle: configuration plus per-mesh coefficient arrays and scratch.
+struct fvx_disc_s {
+	mutable fvx::Profiler prof;
+	fvx_mesh_s* m = nullptr;
+	fvx_physics_cfg pc;
+	fvx_numerics_cfg nc;
+	fvx::GasParams gas;
+	fvx::BCTable bct;
+	std::vector<int> bcid;                 // per boundary face: row of bct
+	fvx::DBuf<int> d_bcid;                 // [nbface]
+	fvx::DBuf<double> d_gcoef;             // [nelem][4][2] gradient stencil coefficients
+	fvx::DBuf<double> d_eps2;              // [nelem] Venkatakrishnan (K h)^3
+	fvx::DBuf<double> d_grad;              // [nelem][8] unlimited primitive gradients
+	fvx::DBuf<double> d_rgrad;             // [nelem][8] reconstruction gradients (limited)
+	fvx::DBuf<double> d_scratch;           // reductions for functionals
+};
+
+namespace fvx {
+void build_mesh(fvx_mesh_s& m, const fvx_mesh_desc& d);
+void rcm_order(const fvx_mesh_desc& d, int* perm);
+void build_disc(fvx_disc_s& d);
+
+// kernel launchers (residual.hip)
+void launch_gradients(const fvx_disc_s& d, const double* u, double* grad, double* rgrad,
+                      int conserved, hipStream_t s);
+void launch_residual(const fvx_disc_s& d, const double* u, double* r, int gettimesteps,
+                     double* dt, hipStream_t s);
+void launch_functionals(const fvx_disc_s& d, const double* u, const double* cgrad, int walltag,
+                        double* dev_out, hipStream_t s);
+int functional_blocks();
+}
+
+#endif

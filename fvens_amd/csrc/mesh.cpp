@@ -1,0 +1,383 @@
+/* mesh.cpp -- host-side mesh preprocessing for libfvx: topology, geometry, RCM ordering and
+ * the per-cell stencil coefficients consumed by the device kernels.
+ *
+ * Topology and numbering follow UMesh (src/mesh/mesh.cpp): faces are numbered physical
+ * boundary first (input order, orientation corrected to point out of the domain, :54-82 and
+ * :683-700), then interior faces in element order with the lower cell on the left
+ * (:709-733). Geometry: normals n = (dy, -dx)/len (:346-365), shoelace areas with quads as
+ * two triangles (:290-313), vertex-average cell centres (:316-328), face midpoints and ghost
+ * centres reflected about the midpoint (src/spatial/aspatial.cpp:50-119).
+ */
+#include "fvx_internal.hpp"
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <queue>
+#include <tuple>
+
+namespace fvx {
+
+namespace {
+
+struct EdgeRec {
+	long long key;
+	int elem, local;
+	bool operator<(const EdgeRec& o) const {
+		return std::tie(key, elem, local) < std::tie(o.key, o.elem, o.local);
+	}
+};
+
+long long ekey(int a, int b)
+{
+	const long long lo = std::min(a, b), hi = std::max(a, b);
+	return (lo << 32) | hi;
+}
+
+void check_desc(const fvx_mesh_desc& d)
+{
+	if(d.npoin <= 0 || d.nelem <= 0 || d.nbface < 0 || !d.coords || !d.inpoel || !d.nnode
+	   || (d.nbface > 0 && (!d.bface || !d.btag)))
+		throw Error(FVX_ERR_INVALID, "mesh description has missing arrays or bad sizes");
+	for(int e = 0; e < d.nelem; e++) {
+		const int nn = d.nnode[e];
+		if(nn != 3 && nn != 4)
+			throw Error(FVX_ERR_INVALID, "cells must be triangles or quadrilaterals");
+		for(int j = 0; j < nn; j++) {
+			const int p = d.inpoel[e*4+j];
+			if(p < 0 || p >= d.npoin) throw Error(FVX_ERR_INVALID, "cell node out of range");
+		}
+	}
+	for(int b = 0; b < d.nbface; b++)
+		for(int j = 0; j < 2; j++) {
+			const int p = d.bface[b*2+j];
+			if(p < 0 || p >= d.npoin) throw Error(FVX_ERR_INVALID, "boundary node out of range");
+		}
+}
+
+/// Sorted list of all cell edges; matching keys are the cell pairs sharing an edge.
+std::vector<EdgeRec> cell_edges(const fvx_mesh_desc& d)
+{
+	std::vector<EdgeRec> e;
+	e.reserve((size_t)d.nelem*4);
+	for(int c = 0; c < d.nelem; c++) {
+		const int nn = d.nnode[c];
+		for(int j = 0; j < nn; j++)
+			e.push_back({ekey(d.inpoel[c*4+j], d.inpoel[c*4+(j+1)%nn]), c, j});
+	}
+	std::sort(e.begin(), e.end());
+	return e;
+}
+
+std::vector<int> neighbours(const fvx_mesh_desc& d, const std::vector<EdgeRec>& e)
+{
+	std::vector<int> nb((size_t)d.nelem*4, -1);
+	for(size_t i = 0; i < e.size(); ) {
+		size_t k = i + 1;
+		while(k < e.size() && e[k].key == e[i].key) k++;
+		if(k - i == 2) {
+			nb[(size_t)e[i].elem*4 + e[i].local] = e[i+1].elem;
+			nb[(size_t)e[i+1].elem*4 + e[i+1].local] = e[i].elem;
+		}
+		else if(k - i > 2)
+			throw Error(FVX_ERR_INVALID, "non-manifold edge shared by more than two cells");
+		i = k;
+	}
+	return nb;
+}
+
+}
+
+void build_mesh(fvx_mesh_s& m, const fvx_mesh_desc& d)
+{
+	check_desc(d);
+	m.npoin = d.npoin; m.nelem = d.nelem; m.nbface = d.nbface;
+	m.coords.assign(d.coords, d.coords + (size_t)d.npoin*2);
+	m.inpoel.assign(d.inpoel, d.inpoel + (size_t)d.nelem*4);
+	m.nnode.assign(d.nnode, d.nnode + d.nelem);
+	m.bface.assign(d.bface, d.bface + (size_t)d.nbface*2);
+	m.btag.assign(d.btag, d.btag + d.nbface);
+
+	const std::vector<EdgeRec> edges = cell_edges(d);
+	m.esuel = neighbours(d, edges);
+	m.elemface.assign((size_t)m.nelem*4, -1);
+
+	// boundary faces: host cell by edge lookup, orientation as the host cell's edge
+	std::vector<int> host(m.nbface), hostj(m.nbface);
+	for(int b = 0; b < m.nbface; b++) {
+		const long long k = ekey(m.bface[b*2], m.bface[b*2+1]);
+		auto it = std::lower_bound(edges.begin(), edges.end(), EdgeRec{k, -1, -1});
+		if(it == edges.end() || it->key != k)
+			throw Error(FVX_ERR_INVALID, "boundary face is not an edge of any cell");
+		if(m.esuel[(size_t)it->elem*4 + it->local] != -1)
+			throw Error(FVX_ERR_INVALID, "boundary face lies between two cells");
+		host[b] = it->elem; hostj[b] = it->local;
+		const int nn = m.nnode[it->elem];
+		m.bface[b*2] = m.inpoel[it->elem*4 + it->local];
+		m.bface[b*2+1] = m.inpoel[it->elem*4 + (it->local+1)%nn];
+	}
+
+	m.ninface = 0;
+	for(int c = 0; c < m.nelem; c++)
+		for(int j = 0; j < m.nnode[c]; j++) {
+			const int o = m.esuel[(size_t)c*4+j];
+			if(o > c) m.ninface++;
+		}
+	m.naface = m.nbface + m.ninface;
+	m.intfac.assign((size_t)m.naface*4, -1);
+	for(int b = 0; b < m.nbface; b++) {
+		m.intfac[b*4+0] = host[b];
+		m.intfac[b*4+1] = m.nelem + b;
+		m.intfac[b*4+2] = m.bface[b*2];
+		m.intfac[b*4+3] = m.bface[b*2+1];
+		m.esuel[(size_t)host[b]*4 + hostj[b]] = m.nelem + b;
+		m.elemface[(size_t)host[b]*4 + hostj[b]] = b;
+	}
+	for(int c = 0; c < m.nelem; c++)
+		for(int j = 0; j < m.nnode[c]; j++)
+			if(m.esuel[(size_t)c*4+j] == -1)
+				throw Error(FVX_ERR_INVALID, "open cell edge without a boundary face");
+	int f = m.nbface;
+	for(int c = 0; c < m.nelem; c++) {
+		const int nn = m.nnode[c];
+		for(int j = 0; j < nn; j++) {
+			const int o = m.esuel[(size_t)c*4+j];
+			if(o > c && o < m.nelem) {
+				m.intfac[(size_t)f*4+0] = c;
+				m.intfac[(size_t)f*4+1] = o;
+				m.intfac[(size_t)f*4+2] = m.inpoel[c*4+j];
+				m.intfac[(size_t)f*4+3] = m.inpoel[c*4+(j+1)%nn];
+				m.elemface[(size_t)c*4+j] = f;
+				for(int k = 0; k < m.nnode[o]; k++)
+					if(m.esuel[(size_t)o*4+k] == c) m.elemface[(size_t)o*4+k] = f;
+				f++;
+			}
+		}
+	}
+
+	const double* X = m.coords.data();
+	m.facemetric.assign((size_t)m.naface*3, 0.0);
+	m.gr.assign((size_t)m.naface*2, 0.0);
+	for(int i = 0; i < m.naface; i++) {
+		const int a = m.intfac[(size_t)i*4+2], b = m.intfac[(size_t)i*4+3];
+		const double nx = X[b*2+1] - X[a*2+1], ny = -(X[b*2] - X[a*2]);
+		const double len = std::sqrt(nx*nx + ny*ny);
+		m.facemetric[(size_t)i*3] = nx/len;
+		m.facemetric[(size_t)i*3+1] = ny/len;
+		m.facemetric[(size_t)i*3+2] = len;
+		m.gr[(size_t)i*2] = (X[a*2] + X[b*2])/2;
+		m.gr[(size_t)i*2+1] = (X[a*2+1] + X[b*2+1])/2;
+	}
+	auto tri = [&](int p, int q, int r) {
+		return 0.5*(X[p*2]*(X[q*2+1] - X[r*2+1]) - X[p*2+1]*(X[q*2] - X[r*2])
+		            + X[q*2]*X[r*2+1] - X[r*2]*X[q*2+1]);
+	};
+	m.area.assign(m.nelem, 0.0);
+	m.rc.assign((size_t)m.nelem*2, 0.0);
+	m.hlong.assign(m.nelem, 0.0);
+	for(int c = 0; c < m.nelem; c++) {
+		const int* in = &m.inpoel[c*4];
+		const int nn = m.nnode[c];
+		m.area[c] = nn == 3 ? tri(in[0], in[1], in[2]) : tri(in[0], in[1], in[2]) + tri(in[0], in[2], in[3]);
+		if(!(m.area[c] > 0)) throw Error(FVX_ERR_INVALID, "cell with non-positive area (not CCW?)");
+		double sx = 0, sy = 0, h2 = 0;
+		for(int j = 0; j < nn; j++) {
+			sx += X[in[j]*2]; sy += X[in[j]*2+1];
+			const int q = in[(j+1)%nn];
+			h2 = std::max(h2, (X[in[j]*2]-X[q*2])*(X[in[j]*2]-X[q*2]) + (X[in[j]*2+1]-X[q*2+1])*(X[in[j]*2+1]-X[q*2+1]));
+		}
+		m.rc[c*2] = sx/nn; m.rc[c*2+1] = sy/nn;
+		m.hlong[c] = std::sqrt(h2);
+	}
+	m.rcbp.assign((size_t)m.nbface*2, 0.0);
+	for(int b = 0; b < m.nbface; b++)
+		for(int k = 0; k < 2; k++)
+			m.rcbp[b*2+k] = 2.0*m.gr[b*2+k] - m.rc[host[b]*2+k];
+
+	// device arrays: per-cell slots and face records
+	std::vector<int> nbr((size_t)m.nelem*4, SLOT_UNUSED), fidx((size_t)m.nelem*4, 0);
+	for(int c = 0; c < m.nelem; c++)
+		for(int j = 0; j < m.nnode[c]; j++) {
+			const int o = m.esuel[(size_t)c*4+j];
+			nbr[(size_t)c*4+j] = o < m.nelem ? o : -1 - (o - m.nelem);
+			fidx[(size_t)c*4+j] = m.elemface[(size_t)c*4+j];
+		}
+	std::vector<FaceGeom> fg(m.naface);
+	for(int i = 0; i < m.naface; i++)
+		fg[i] = FaceGeom{m.facemetric[(size_t)i*3], m.facemetric[(size_t)i*3+1],
+		                 m.facemetric[(size_t)i*3+2], m.gr[(size_t)i*2], m.gr[(size_t)i*2+1], 0.0};
+	m.d_nbr.upload(nbr);
+	m.d_face.upload(fidx);
+	m.d_fgeom.upload(fg);
+	m.d_centre.upload(m.rc);
+	m.d_area.upload(m.area);
+	m.d_bhost.upload(host);
+	m.d_btag.upload(m.btag);
+}
+
+/* Reverse Cuthill-McKee on the cell adjacency graph, started from a pseudo-peripheral cell
+ * (George-Liu), neighbours visited by increasing degree. */
+void rcm_order(const fvx_mesh_desc& d, int* perm)
+{
+	check_desc(d);
+	const int n = d.nelem;
+	const std::vector<int> nb = neighbours(d, cell_edges(d));
+	std::vector<int> deg(n, 0);
+	for(int c = 0; c < n; c++)
+		for(int j = 0; j < 4; j++) if(nb[(size_t)c*4+j] >= 0) deg[c]++;
+	std::vector<int> level(n, -1), order;
+	order.reserve(n);
+	std::vector<char> done(n, 0);
+	auto bfs_levels = [&](int s, int& last_min) {
+		std::fill(level.begin(), level.end(), -1);
+		std::queue<int> q;
+		q.push(s); level[s] = 0;
+		int maxl = 0;
+		last_min = s;
+		while(!q.empty()) {
+			const int c = q.front(); q.pop();
+			if(level[c] > maxl || (level[c] == maxl && deg[c] < deg[last_min])) {
+				maxl = level[c]; last_min = c;
+			}
+			for(int j = 0; j < 4; j++) {
+				const int o = nb[(size_t)c*4+j];
+				if(o >= 0 && level[o] < 0) { level[o] = level[c] + 1; q.push(o); }
+			}
+		}
+		return maxl;
+	};
+	for(int seed = 0; seed < n; seed++) {
+		if(done[seed]) continue;
+		// lowest-degree cell of this component as the first guess
+		int s = seed;
+		{
+			std::queue<int> q;
+			q.push(seed);
+			level.assign(n, -1); level[seed] = 0;
+			while(!q.empty()) {
+				const int c = q.front(); q.pop();
+				if(deg[c] < deg[s]) s = c;
+				for(int j = 0; j < 4; j++) {
+					const int o = nb[(size_t)c*4+j];
+					if(o >= 0 && level[o] < 0) { level[o] = 0; q.push(o); }
+				}
+			}
+		}
+		int far = s;
+		int ecc = bfs_levels(s, far);
+		for(int it = 0; it < 8; it++) {
+			int far2 = far;
+			const int e2 = bfs_levels(far, far2);
+			if(e2 <= ecc) break;
+			s = far; ecc = e2; far = far2;
+		}
+		std::queue<int> q;
+		q.push(s); done[s] = 1;
+		while(!q.empty()) {
+			const int c = q.front(); q.pop();
+			order.push_back(c);
+			int cand[4], nc = 0;
+			for(int j = 0; j < 4; j++) {
+				const int o = nb[(size_t)c*4+j];
+				if(o >= 0 && !done[o]) { cand[nc++] = o; done[o] = 1; }
+			}
+			std::sort(cand, cand + nc, [&](int a, int b) {
+				return deg[a] != deg[b] ? deg[a] < deg[b] : a < b; });
+			for(int k = 0; k < nc; k++) q.push(cand[k]);
+		}
+	}
+	for(int i = 0; i < n; i++) perm[i] = order[n-1-i];
+}
+
+/* Stencil coefficients: both gradient schemes become grad_i = sum_j c_ij (w_j - w_i) over the
+ * cell's face neighbours (ghost centre for boundary faces).
+ *  - weighted least squares (src/spatial/agradientschemes.cpp:218-440):
+ *        c_ij = w_ij V_i^{-1} (x_j - x_i),  w_ij = |x_j - x_i|^-2,  V_i = sum_j w_ij dx dx^T
+ *  - Green-Gauss with inverse-distance face values (:61-214):
+ *        c_ij = len_f n_out,f (d_j/(d_i+d_j)) / A_i,  d = 1/|x_f - x|
+ *    (the u_i term carries sum_f len_f n_out,f = 0 for a closed cell). */
+void build_disc(fvx_disc_s& d)
+{
+	const fvx_mesh_s& m = *d.m;
+	const fvx_physics_cfg& p = d.pc;
+	GasParams& g = d.gas;
+	g.g = p.gamma; g.gm1 = p.gamma - 1.0;
+	g.Minf = p.Minf; g.Tinf = p.Tinf; g.Reinf = p.Reinf; g.Pr = p.Pr;
+	g.gM2 = p.gamma*p.Minf*p.Minf;
+	g.sutherland = 110.5/p.Tinf;
+	g.pinf = 1.0/(p.gamma*p.Minf*p.Minf);
+	g.uinf[0] = 1.0;
+	g.uinf[1] = std::cos(p.aoa);
+	g.uinf[2] = std::sin(p.aoa);
+	g.uinf[3] = g.pinf/(p.gamma - 1.0) + 0.5;
+	g.constvisc = p.const_visc;
+
+	if(p.nbc < 0 || p.nbc > 8) throw Error(FVX_ERR_INVALID, "at most 8 boundary conditions");
+	for(int k = 0; k < 8; k++) { d.bct.kind[k] = -1; d.bct.v0[k] = 0; d.bct.v1[k] = 0; }
+	for(int k = 0; k < p.nbc; k++) {
+		const int kind = p.bcs[k].kind;
+		if(kind < 0 || kind > 7 || kind == FVX_BC_PERIODIC)
+			throw Error(FVX_ERR_UNSUPPORTED, "boundary kind not supported");
+		d.bct.kind[k] = kind; d.bct.v0[k] = p.bcs[k].vals[0]; d.bct.v1[k] = p.bcs[k].vals[1];
+	}
+	d.bcid.assign(m.nbface, -1);
+	for(int b = 0; b < m.nbface; b++) {
+		for(int k = 0; k < p.nbc; k++) if(p.bcs[k].tag == m.btag[b]) d.bcid[b] = k;
+		if(d.bcid[b] < 0) throw Error(FVX_ERR_INVALID, "boundary marker without a boundary condition");
+	}
+	d.d_bcid.upload(d.bcid);
+
+	const fvx_numerics_cfg& n = d.nc;
+	if(n.conv_numflux < 0 || n.conv_numflux > 6 || n.conv_numflux_jac < 0 || n.conv_numflux_jac > 6)
+		throw Error(FVX_ERR_INVALID, "unknown inviscid flux");
+	if(n.gradientscheme < 0 || n.gradientscheme > 2 || n.reconstruction < 0 || n.reconstruction > 4)
+		throw Error(FVX_ERR_INVALID, "unknown gradient scheme or reconstruction");
+	if(n.reconstruction == FVX_RECON_WENO)
+		throw Error(FVX_ERR_UNSUPPORTED, "WENO reconstruction is not built on the device");
+
+	const int ne = m.nelem;
+	std::vector<double> coef((size_t)ne*8, 0.0), eps2(ne, 0.0);
+	for(int c = 0; c < ne; c++) {
+		const double xi = m.rc[c*2], yi = m.rc[c*2+1];
+		double xj[4], yj[4];
+		int ns = m.nnode[c];
+		for(int j = 0; j < ns; j++) {
+			const int o = m.esuel[(size_t)c*4+j];
+			if(o < ne) { xj[j] = m.rc[o*2]; yj[j] = m.rc[o*2+1]; }
+			else { xj[j] = m.rcbp[(o-ne)*2]; yj[j] = m.rcbp[(o-ne)*2+1]; }
+		}
+		if(n.gradientscheme == FVX_GRAD_LEASTSQUARES) {
+			double V[4] = {0, 0, 0, 0};
+			for(int j = 0; j < ns; j++) {
+				const double dx = xj[j]-xi, dy = yj[j]-yi, w = 1.0/(dx*dx + dy*dy);
+				V[0] += w*dx*dx; V[1] += w*dx*dy; V[2] += w*dy*dx; V[3] += w*dy*dy;
+			}
+			const double det = V[0]*V[3] - V[1]*V[2];
+			const double I[4] = { V[3]/det, -V[1]/det, -V[2]/det, V[0]/det };
+			for(int j = 0; j < ns; j++) {
+				const double dx = xj[j]-xi, dy = yj[j]-yi, w = 1.0/(dx*dx + dy*dy);
+				coef[(size_t)c*8 + j*2]     = w*(I[0]*dx + I[1]*dy);
+				coef[(size_t)c*8 + j*2 + 1] = w*(I[2]*dx + I[3]*dy);
+			}
+		}
+		else if(n.gradientscheme == FVX_GRAD_GREENGAUSS) {
+			for(int j = 0; j < ns; j++) {
+				const int f = m.elemface[(size_t)c*4+j];
+				const double s = m.intfac[(size_t)f*4] == c ? 1.0 : -1.0;
+				const double mx = m.gr[f*2], my = m.gr[f*2+1];
+				const double di = 1.0/std::sqrt((mx-xi)*(mx-xi) + (my-yi)*(my-yi));
+				const double dj = 1.0/std::sqrt((mx-xj[j])*(mx-xj[j]) + (my-yj[j])*(my-yj[j]));
+				const double wgt = dj/(di + dj) * m.facemetric[(size_t)f*3+2] / m.area[c] * s;
+				coef[(size_t)c*8 + j*2]     = wgt*m.facemetric[(size_t)f*3];
+				coef[(size_t)c*8 + j*2 + 1] = wgt*m.facemetric[(size_t)f*3+1];
+			}
+		}
+		eps2[c] = std::pow(n.limiter_param*m.hlong[c], 3);
+	}
+	d.d_gcoef.upload(coef);
+	d.d_eps2.upload(eps2);
+	d.d_grad.alloc((size_t)ne*8);
+	d.d_rgrad.alloc((size_t)ne*8);
+	d.d_scratch.alloc(4096);
+}
+
+}

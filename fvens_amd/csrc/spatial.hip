@@ -1,0 +1,277 @@
+/* spatial.hip -- gradient / limiter kernel, residual dispatch and output functionals.
+ *
+ * Gradients (GradientScheme::compute_gradients, src/spatial/agradientschemes.cpp:61-440) are
+ * evaluated as precomputed face stencils grad_i = sum_j c_ij (w_j - w_i) over the cell's
+ * neighbours (ghost state of the cell value for a boundary face), one face per lane, summed
+ * by a quad reduction. The Venkatakrishnan and Barth-Jespersen limiters
+ * (src/spatial/limitedlinearreconstruction.cpp:116-268) are fused into the same pass: the
+ * neighbour differences that form the gradient also give the max/min bounds, and the limited
+ * gradient phi*grad is what the flux pass reads. Boundary neighbours contribute the ghost
+ * state (the reference reads past its cell array there; this definition is deliberate).
+ */
+#include "fvx_internal.hpp"
+#include "wave.hpp"
+
+namespace fvx {
+
+#define FVX_DECL_FLUX(ID) void launch_flux_##ID(int mode, bool visc, int ncell, const double* u, \
+	const double* rgrad, const double* grad, const int* nbr, const int* face, const FaceGeom* fg, \
+	const double* centre, const double* area, const int* bcid, const GasParams& gp, \
+	const BCTable& bt, double* res, double* dt, int getdt, hipStream_t s);
+FVX_DECL_FLUX(0) FVX_DECL_FLUX(1) FVX_DECL_FLUX(2) FVX_DECL_FLUX(3) FVX_DECL_FLUX(4)
+FVX_DECL_FLUX(5) FVX_DECL_FLUX(6)
+#undef FVX_DECL_FLUX
+
+/// LIM: 0 none, 1 Venkatakrishnan, 2 Barth-Jespersen. CONS: gradients of the conserved
+/// variables (getGradients) instead of the primitive ones. gout receives the unlimited
+/// gradient (may be null when LIM > 0 and it is not needed), rout the limited one.
+template <int LIM, bool CONS>
+__global__ __launch_bounds__(BLOCK)
+void k_gradients(int ncell, const double* __restrict__ u, const int* __restrict__ nbr,
+                 const int* __restrict__ face, const FaceGeom* __restrict__ fgeom,
+                 const double* __restrict__ centre, const double* __restrict__ gcoef,
+                 const double* __restrict__ eps2, const int* __restrict__ bcid,
+                 const GasParams gp, const BCTable bt, double* __restrict__ gout,
+                 double* __restrict__ rout)
+{
+	const int q = threadIdx.x & 3;
+	const int cell = xcd_block()*CELLS_PER_BLOCK + (threadIdx.x >> 2);
+	const bool valid = cell < ncell;
+	const int cc = valid ? cell : 0;
+	const int nb = nbr[cc*4 + q];
+	const bool act = valid && nb != SLOT_UNUSED;
+
+	double ui[4], wi[4];
+	load4(u + 4*cc, ui);
+	if constexpr(CONS) { for(int k = 0; k < 4; k++) wi[k] = ui[k]; }
+	else to_primitive(gp, ui, wi);
+
+	double dw[4] = {0.0, 0.0, 0.0, 0.0};
+	double cx = 0.0, cy = 0.0;
+	FaceGeom G{};
+	if(act) {
+		G = fgeom[face[cc*4 + q]];
+		double uj[4], wj[4];
+		if(nb >= 0) load4(u + 4*nb, uj);
+		else {
+			const double n[2] = { G.nx, G.ny };
+			ghost_state(gp, bt, bcid[-1 - nb], ui, n, uj);
+		}
+		if constexpr(CONS) { for(int k = 0; k < 4; k++) wj[k] = uj[k]; }
+		else to_primitive(gp, uj, wj);
+		for(int k = 0; k < 4; k++) dw[k] = wj[k] - wi[k];
+		const double2 c = reinterpret_cast<const double2*>(gcoef)[cc*4 + q];
+		cx = c.x; cy = c.y;
+	}
+	double g[8];
+	for(int k = 0; k < 4; k++) {
+		g[2*k] = quad_sum(cx*dw[k]);
+		g[2*k+1] = quad_sum(cy*dw[k]);
+	}
+
+	if constexpr(LIM == 0) {
+		if(valid && q < 2) store4(gout + 8*cc + 4*q, g + 4*q);
+	} else {
+		double phi[4];
+		const double xi = centre[2*cc], yi = centre[2*cc+1];
+		const double e2 = LIM == 1 ? eps2[cc] : 0.0;
+		for(int k = 0; k < 4; k++) {
+			const double dmax = quad_max(act ? fmax(dw[k], 0.0) : 0.0);
+			const double dmin = quad_min(act ? fmin(dw[k], 0.0) : 0.0);
+			double ph = 1.0;
+			if(act) {
+				// face value by linear extrapolation, then its difference from the cell value
+				double uf = wi[k];
+				uf += g[2*k]*(G.mx - xi);
+				uf += g[2*k+1]*(G.my - yi);
+				const double dm = uf - wi[k];
+				if constexpr(LIM == 1) {
+					const double dp = dm < 0 ? dmin : dmax;
+					ph = (dp*dp + 2*dp*dm + e2)/(dp*dp + dp*dm + 2*dm*dm + e2);
+				} else {
+					if(dm > 0) ph = 1 < dmax/dm ? 1.0 : dmax/dm;
+					else if(dm < 0) ph = 1 < dmin/dm ? 1.0 : dmin/dm;
+				}
+			}
+			phi[k] = fmin(quad_min(ph), 1.0);
+		}
+		double lg[8];
+		for(int k = 0; k < 4; k++) { lg[2*k] = phi[k]*g[2*k]; lg[2*k+1] = phi[k]*g[2*k+1]; }
+		if(valid) {
+			if(q < 2) { if(gout) store4(gout + 8*cc + 4*q, g + 4*q); }
+			else store4(rout + 8*cc + 4*(q-2), lg + 4*(q-2));
+		}
+	}
+}
+
+static int grid_of(int ncell) { return (ncell + CELLS_PER_BLOCK - 1)/CELLS_PER_BLOCK; }
+
+/* ------------------------------------------------------------------ profiler ----------- */
+
+void Profiler::start(int k, hipStream_t s)
+{
+	if(!on) return;
+	if(2*used + 2 > pool.size()) {
+		for(int i = 0; i < 2; i++) {
+			hipEvent_t e;
+			FVX_HIP(hipEventCreate(&e));
+			pool.push_back(e);
+		}
+		kid.push_back(0);
+	}
+	kid[used] = k;
+	FVX_HIP(hipEventRecord(pool[2*used], s));
+}
+
+void Profiler::stop(hipStream_t s)
+{
+	if(!on) return;
+	FVX_HIP(hipEventRecord(pool[2*used+1], s));
+	used++;
+}
+
+void Profiler::collect()
+{
+	for(size_t i = 0; i < used; i++) {
+		FVX_HIP(hipEventSynchronize(pool[2*i+1]));
+		float ms = 0;
+		FVX_HIP(hipEventElapsedTime(&ms, pool[2*i], pool[2*i+1]));
+		total_ms[kid[i]] += ms;
+		count[kid[i]]++;
+	}
+	used = 0;
+}
+
+void Profiler::reset()
+{
+	used = 0;
+	for(int k = 0; k < NKERNEL; k++) { total_ms[k] = 0; count[k] = 0; }
+}
+
+Profiler::~Profiler()
+{
+	for(hipEvent_t e : pool) (void)hipEventDestroy(e);
+}
+
+void launch_gradients(const fvx_disc_s& d, const double* u, double* grad, double* rgrad,
+                      int conserved, hipStream_t s)
+{
+	const fvx_mesh_s& m = *d.m;
+	const int nblk = grid_of(m.nelem);
+	const int lim = conserved ? 0 : (d.nc.reconstruction == FVX_RECON_VENKATAKRISHNAN ? 1 :
+	                                 d.nc.reconstruction == FVX_RECON_BARTHJESPERSEN ? 2 : 0);
+#define FVX_GL(L, C) hipLaunchKernelGGL((k_gradients<L, C>), dim3(nblk), dim3(BLOCK), 0, s, \
+		m.nelem, u, m.d_nbr.p, m.d_face.p, m.d_fgeom.p, m.d_centre.p, d.d_gcoef.p, d.d_eps2.p, \
+		d.d_bcid.p, d.gas, d.bct, grad, rgrad)
+	d.prof.start(PROF_GRADIENTS, s);
+	if(conserved) FVX_GL(0, true);
+	else if(lim == 0) FVX_GL(0, false);
+	else if(lim == 1) FVX_GL(1, false);
+	else FVX_GL(2, false);
+#undef FVX_GL
+	FVX_HIP(hipGetLastError());
+	d.prof.stop(s);
+}
+
+void launch_residual(const fvx_disc_s& d, const double* u, double* r, int gettimesteps,
+                     double* dt, hipStream_t s)
+{
+	const fvx_mesh_s& m = *d.m;
+	const bool order2 = d.nc.order2 && d.nc.gradientscheme != FVX_GRAD_NONE;
+	const bool visc = d.pc.viscous != 0;
+	const int rec = d.nc.reconstruction;
+	int mode = 0;
+	const double* rg = nullptr;
+	const double* ug = nullptr;
+	if(order2) {
+		const bool limited = rec == FVX_RECON_VENKATAKRISHNAN || rec == FVX_RECON_BARTHJESPERSEN;
+		if(limited) {
+			launch_gradients(d, u, visc ? d.d_grad.p : nullptr, d.d_rgrad.p, 0, s);
+			rg = d.d_rgrad.p; ug = d.d_grad.p; mode = 1;
+		} else {
+			launch_gradients(d, u, d.d_grad.p, nullptr, 0, s);
+			rg = d.d_grad.p; ug = d.d_grad.p;
+			mode = rec == FVX_RECON_VANALBADA ? 2 : 1;
+		}
+	}
+	d.prof.start(PROF_FLUX, s);
+	switch(d.nc.conv_numflux) {
+#define FVX_C(ID) case ID: launch_flux_##ID(mode, visc, m.nelem, u, rg, ug, m.d_nbr.p, \
+	m.d_face.p, m.d_fgeom.p, m.d_centre.p, m.d_area.p, d.d_bcid.p, d.gas, d.bct, r, dt, \
+	gettimesteps, s); break;
+	FVX_C(0) FVX_C(1) FVX_C(2) FVX_C(3) FVX_C(4) FVX_C(5) FVX_C(6)
+#undef FVX_C
+	default: throw Error(FVX_ERR_INVALID, "unknown flux");
+	}
+	FVX_HIP(hipGetLastError());
+	d.prof.stop(s);
+}
+
+/* ---------------------------------------------------------------- functionals ---------- */
+
+constexpr int FN_BLOCKS = 128;
+
+/// Per-block partial sums: [0] CL, [1] CDp, [2] CDsf, [3] wall length, [4] entropy error^2.
+/// Restates computeSurfaceData (src/spatial/flow_spatial.cpp:130-310, cell-centred states,
+/// normalised by wall length) and compute_entropy_cell (src/spatial/aoutput.cpp:28-62).
+__global__ __launch_bounds__(BLOCK)
+void k_functionals(int ncell, int nbface, const double* __restrict__ u,
+                   const double* __restrict__ cgrad, const int* __restrict__ bhost,
+                   const int* __restrict__ btag, const FaceGeom* __restrict__ fgeom,
+                   const double* __restrict__ area, int walltag, GasParams gp, double cosa,
+                   double sina, double* __restrict__ part)
+{
+	__shared__ double sh[BLOCK][5];
+	double acc[5] = {0, 0, 0, 0, 0};
+	const double sref = pressure(gp, gp.uinf)/pow(gp.uinf[0], gp.g);
+	for(int f = blockIdx.x*BLOCK + threadIdx.x; f < nbface; f += gridDim.x*BLOCK) {
+		if(btag[f] != walltag) continue;
+		const int c = bhost[f];
+		const FaceGeom G = fgeom[f];
+		double w[4];
+		load4(u + 4*c, w);
+		const double cp = (pressure(gp, w) - gp.pinf)*2.0;
+		const double mu = viscosity(gp, w);
+		double gu[2][2];
+		for(int i = 0; i < 2; i++)
+			for(int j = 0; j < 2; j++)
+				gu[i][j] = (cgrad[8*c + (i+1)*2 + j]*w[0] - w[i+1]*cgrad[8*c + j])/(w[0]*w[0]);
+		const double n[2] = { G.nx, G.ny }, t[2] = { G.ny, -G.nx };
+		double fo[2];
+		for(int i = 0; i < 2; i++) fo[i] = (gu[i][0] + gu[0][i])*n[0] + (gu[i][1] + gu[1][i])*n[1];
+		const double csf = 2*mu*(fo[0]*t[0] + fo[1]*t[1]);
+		acc[0] += cp*(n[0]*(-sina) + n[1]*cosa)*G.len;
+		acc[1] += cp*(n[0]*cosa + n[1]*sina)*G.len;
+		acc[2] += csf*(t[0]*cosa + t[1]*sina)*G.len;
+		acc[3] += G.len;
+	}
+	for(int c = blockIdx.x*BLOCK + threadIdx.x; c < ncell; c += gridDim.x*BLOCK) {
+		double w[4];
+		load4(u + 4*c, w);
+		const double s = pressure(gp, w)/pow(w[0], gp.g);
+		const double e = (s - sref)/sref;
+		acc[4] += e*e*area[c];
+	}
+	for(int k = 0; k < 5; k++) sh[threadIdx.x][k] = acc[k];
+	__syncthreads();
+	for(int st = BLOCK/2; st > 0; st >>= 1) {
+		if(threadIdx.x < st)
+			for(int k = 0; k < 5; k++) sh[threadIdx.x][k] += sh[threadIdx.x + st][k];
+		__syncthreads();
+	}
+	if(threadIdx.x < 5) part[blockIdx.x*5 + threadIdx.x] = sh[0][threadIdx.x];
+}
+
+void launch_functionals(const fvx_disc_s& d, const double* u, const double* cgrad, int walltag,
+                        double* dev_out, hipStream_t s)
+{
+	const fvx_mesh_s& m = *d.m;
+	hipLaunchKernelGGL(k_functionals, dim3(FN_BLOCKS), dim3(BLOCK), 0, s, m.nelem, m.nbface, u,
+	                   cgrad, m.d_bhost.p, m.d_btag.p, m.d_fgeom.p, m.d_area.p, walltag, d.gas,
+	                   cos(d.pc.aoa), sin(d.pc.aoa), dev_out);
+	FVX_HIP(hipGetLastError());
+}
+
+int functional_blocks() { return FN_BLOCKS; }
+
+}

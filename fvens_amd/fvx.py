@@ -1,0 +1,227 @@
+"""ctypes binding of libfvx (include/fvx.h) for Python callers, tests and the benchmark.
+
+Device buffers are torch tensors on the current HIP device; only their data pointers and the
+current stream cross the C boundary. The library is required: importing this module on a
+machine without a built libfvx.so raises immediately instead of falling back to anything.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from .meshgen import RawMesh
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfvx.so")
+
+FLUX = {"ROE": 0, "HLLC": 1, "HLL": 2, "LLF": 3, "AUSM": 4, "AUSMPLUS": 5, "VANLEER": 6}
+GRAD = {"NONE": 0, "GREENGAUSS": 1, "LEASTSQUARES": 2}
+RECON = {"NONE": 0, "VANALBADA": 1, "VENKATAKRISHNAN": 2, "BARTHJESPERSEN": 3, "WENO": 4}
+BC = {"slipwall": 0, "farfield": 1, "inflowoutflow": 2, "subsonicinflow": 3,
+      "extrapolation": 4, "periodic": 5, "isothermalwall": 6, "adiabaticwall": 7}
+
+STATUS = {0: "OK", 1: "invalid argument", 2: "HIP error", 3: "non-finite result",
+          4: "not converged", 5: "unsupported", 6: "allocation failed"}
+
+
+class FvxError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"fvx status {code} ({STATUS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+class NumericalError(FvxError):
+    """Non-finite residual (the reference's Numerical_error)."""
+
+
+class ToleranceError(FvxError):
+    """Iteration limit reached (the reference's Tolerance_error)."""
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("npoin", C.c_int), ("nelem", C.c_int), ("nbface", C.c_int),
+                ("coords", C.POINTER(C.c_double)), ("inpoel", C.POINTER(C.c_int)),
+                ("nnode", C.POINTER(C.c_int)), ("bface", C.POINTER(C.c_int)),
+                ("btag", C.POINTER(C.c_int))]
+
+
+class BCCfg(C.Structure):
+    _fields_ = [("tag", C.c_int), ("kind", C.c_int), ("vals", C.c_double * 2)]
+
+
+class PhysicsCfg(C.Structure):
+    _fields_ = [("gamma", C.c_double), ("Minf", C.c_double), ("Tinf", C.c_double),
+                ("Reinf", C.c_double), ("Pr", C.c_double), ("aoa", C.c_double),
+                ("viscous", C.c_int), ("const_visc", C.c_int), ("nbc", C.c_int),
+                ("bcs", BCCfg * 8)]
+
+
+class NumericsCfg(C.Structure):
+    _fields_ = [("conv_numflux", C.c_int), ("conv_numflux_jac", C.c_int),
+                ("gradientscheme", C.c_int), ("reconstruction", C.c_int),
+                ("limiter_param", C.c_double), ("order2", C.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libfvx.so is not built ({LIB_PATH}); run __graft_entry__.build()")
+        L = C.CDLL(LIB_PATH)
+        L.fvx_last_error.restype = C.c_char_p
+        _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        msg = lib().fvx_last_error().decode()
+        cls = NumericalError if rc == 3 else ToleranceError if rc == 4 else FvxError
+        raise cls(rc, msg)
+
+
+def _ptr(a, ct):
+    return a.ctypes.data_as(C.POINTER(ct))
+
+
+def _dptr(t):
+    """Device pointer of a torch tensor (must be contiguous)."""
+    assert t.is_contiguous()
+    return C.c_void_p(t.data_ptr())
+
+
+def _stream():
+    import torch
+    return C.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def physics_cfg(gamma=1.4, Minf=0.5, Tinf=298.0, Reinf=float("inf"), Pr=float("nan"),
+                aoa_deg=0.0, viscous=False, const_visc=False, bcs=()):
+    p = PhysicsCfg()
+    p.gamma, p.Minf, p.Tinf, p.Reinf, p.Pr = gamma, Minf, Tinf, Reinf, Pr
+    p.aoa = float(np.deg2rad(aoa_deg))
+    p.viscous, p.const_visc = int(viscous), int(const_visc)
+    p.nbc = len(bcs)
+    for i, b in enumerate(bcs):
+        p.bcs[i].tag = b[0]
+        p.bcs[i].kind = BC[b[1]]
+        vals = b[2] if len(b) > 2 else (0.0, 0.0)
+        p.bcs[i].vals[0] = vals[0]
+        p.bcs[i].vals[1] = vals[1] if len(vals) > 1 else 0.0
+    return p
+
+
+def numerics_cfg(flux="ROE", jacflux=None, gradient="LEASTSQUARES", recon="NONE",
+                 limiter_param=5.0, order2=True):
+    n = NumericsCfg()
+    n.conv_numflux = FLUX[flux]
+    n.conv_numflux_jac = FLUX[jacflux or flux]
+    n.gradientscheme = GRAD[gradient]
+    n.reconstruction = RECON[recon]
+    n.limiter_param = limiter_param
+    n.order2 = int(order2 and gradient != "NONE")
+    return n
+
+
+def _desc(raw: RawMesh):
+    arrs = dict(coords=np.ascontiguousarray(raw.coords, np.float64),
+                inpoel=np.ascontiguousarray(raw.inpoel, np.int32),
+                nnode=np.ascontiguousarray(raw.nnode, np.int32),
+                bface=np.ascontiguousarray(raw.bface, np.int32),
+                btag=np.ascontiguousarray(raw.btag, np.int32))
+    d = MeshDesc(raw.npoin, raw.nelem, raw.nbface, _ptr(arrs["coords"], C.c_double),
+                 _ptr(arrs["inpoel"], C.c_int), _ptr(arrs["nnode"], C.c_int),
+                 _ptr(arrs["bface"], C.c_int), _ptr(arrs["btag"], C.c_int))
+    return d, arrs
+
+
+def rcm_permutation(raw: RawMesh) -> np.ndarray:
+    """Reverse Cuthill-McKee cell order (perm[new] = old) computed by libfvx."""
+    d, keep = _desc(raw)
+    perm = np.zeros(raw.nelem, np.int32)
+    check(lib().fvx_mesh_rcm(C.byref(d), _ptr(perm, C.c_int)))
+    return perm
+
+
+class Mesh:
+    """Device mesh (UMesh after preprocessing) built by libfvx."""
+
+    def __init__(self, raw: RawMesh):
+        self.raw = raw
+        d, keep = _desc(raw)
+        h = C.c_void_p()
+        check(lib().fvx_mesh_create(C.byref(d), C.byref(h)))
+        self.h = h
+        s = np.zeros(5, np.int32)
+        check(lib().fvx_mesh_sizes(h, _ptr(s, C.c_int)))
+        self.npoin, self.nelem, self.nbface, self.ninface, self.naface = map(int, s)
+
+    def faces(self):
+        intfac = np.zeros((self.naface, 4), np.int32)
+        fm = np.zeros((self.naface, 3))
+        area = np.zeros(self.nelem)
+        rc = np.zeros((self.nelem, 2))
+        check(lib().fvx_mesh_faces(self.h, _ptr(intfac, C.c_int), _ptr(fm, C.c_double),
+                                   _ptr(area, C.c_double), _ptr(rc, C.c_double)))
+        return intfac, fm, area, rc
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.fvx_mesh_destroy(self.h)
+            self.h = None
+
+
+class FlowFV:
+    """Device spatial discretisation with the reference's FlowFV_base operations."""
+
+    def __init__(self, mesh: Mesh, pcfg: PhysicsCfg, ncfg: NumericsCfg):
+        self.mesh = mesh
+        self.pcfg, self.ncfg = pcfg, ncfg
+        h = C.c_void_p()
+        check(lib().fvx_disc_create(mesh.h, C.byref(pcfg), C.byref(ncfg), C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.fvx_disc_destroy(self.h)
+            self.h = None
+
+    def freestream(self) -> np.ndarray:
+        out = np.zeros(4)
+        check(lib().fvx_freestream_state(self.h, _ptr(out, C.c_double)))
+        return out
+
+    def compute_residual(self, u, r, gettimesteps=False, dt=None):
+        """Adds -r(u) into r; fills dt with local time steps if requested (device tensors)."""
+        check(lib().fvx_compute_residual(self.h, _dptr(u), _dptr(r), int(gettimesteps),
+                                         _dptr(dt) if dt is not None else C.c_void_p(0),
+                                         _stream()))
+
+    def get_gradients(self, u, grads):
+        check(lib().fvx_get_gradients(self.h, _dptr(u), _dptr(grads), _stream()))
+
+    def get_primitive_gradients(self, u, grads):
+        check(lib().fvx_get_primitive_gradients(self.h, _dptr(u), _dptr(grads), _stream()))
+
+    KERNELS = {"gradients": 0, "flux": 1, "jacobian": 2, "pc": 3}
+
+    def profile_enable(self, on=True):
+        check(lib().fvx_profile_enable(self.h, int(on)))
+
+    def profile_read(self, kernel):
+        """(total device ms, launches) of one kernel since profiling was enabled."""
+        t = C.c_double()
+        n = C.c_long()
+        check(lib().fvx_profile_read(self.h, self.KERNELS[kernel], C.byref(t), C.byref(n)))
+        return t.value, n.value
+
+    def functionals(self, u, wall_tag=2):
+        out = np.zeros(4)
+        check(lib().fvx_functionals(self.h, _dptr(u), int(wall_tag), _ptr(out, C.c_double),
+                                    _stream()))
+        return dict(CL=out[0], CDp=out[1], CDsf=out[2], entropy=out[3])
