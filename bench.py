@@ -37,7 +37,28 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=12.0,
                    help="bounded CPU-baseline sample (seconds of CPU work)")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--calibrate-copy", action="store_true",
+                   help="run only a plain device-to-device copy of known size (the reference "
+                        "kernel for scaling HBM counter readings)")
     return p.parse_args()
+
+
+def calibrate_copy(steps):
+    """Plain copy of 2^27 doubles (1 GiB read + 1 GiB write per launch)."""
+    import torch
+    n = 1 << 27
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / steps
+    print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8,
+                      "copy_GBs": round(2 * n * 8 / dt / 1e9, 1), "launches": steps + 3}))
 
 
 def build_case():
@@ -82,6 +103,9 @@ def main():
     import torch
     from fvens_amd import fvx
     assert torch.cuda.is_available(), "no GPU"
+    if a.calibrate_copy:
+        calibrate_copy(a.steps)
+        return
     raw, bcs, pcfg, ncfg, u_host = build_case()
     mesh = fvx.Mesh(raw)
     disc = fvx.FlowFV(mesh, pcfg, ncfg)

@@ -22,6 +22,7 @@
 #define FVX_FLUX_KERNEL_HPP
 
 #include "fvx_internal.hpp"
+#include "face_state.hpp"
 #include "wave.hpp"
 
 namespace fvx {
@@ -29,26 +30,28 @@ namespace fvx {
 /// Viscous flux across a face (compute_viscous_flux, flow_spatial.cpp:348-395;
 /// getPrimitive2StatesAndGradients, viscousphysics.cpp:14-68; modified-average face gradient,
 /// aspatial.cpp:172-205; computeViscousFlux, viscousphysics.cpp:70-122).
-/// wl, wr: primitive cell-centred states; gl, gr: primitive gradients [var][dim] (null for
-/// first order); ul, ur: conserved face states.
+/// wl, wr: primitive cell-centred states with reciprocal densities irl, irr; gl, gr: primitive
+/// gradients [var][dim] (null for first order); A, B: face states.
 __device__ __forceinline__ void viscous_flux(const GasParams& gp, const double* n,
                                              const double* xl, const double* xr,
-                                             const double* wl, const double* wr,
+                                             const double* wl, double irl,
+                                             const double* wr, double irr,
                                              const double* gl, const double* gr,
-                                             const double* ul, const double* ur, double* vf)
+                                             const FaceState& A, const FaceState& B, double* vf)
 {
 	// temperature and its gradient from (rho, p) and their gradients
-	const double Tl = wl[3]/wl[0]*gp.gM2, Tr = wr[3]/wr[0]*gp.gM2;
+	const double Tl = wl[3]*irl*gp.gM2, Tr = wr[3]*irr*gp.gM2;
 	double gtl[2] = {0, 0}, gtr[2] = {0, 0};
 	if(gl) {
 		for(int j = 0; j < 2; j++) {
-			gtl[j] = (gl[3*2+j]*wl[0] - wl[3]*gl[0*2+j])/(wl[0]*wl[0])*gp.gM2;
-			gtr[j] = (gr[3*2+j]*wr[0] - wr[3]*gr[0*2+j])/(wr[0]*wr[0])*gp.gM2;
+			gtl[j] = (gl[3*2+j]*wl[0] - wl[3]*gl[0*2+j])*(irl*irl)*gp.gM2;
+			gtr[j] = (gr[3*2+j]*wr[0] - wr[3]*gr[0*2+j])*(irr*irr)*gp.gM2;
 		}
 	}
 	double dr[2] = { xr[0] - xl[0], xr[1] - xl[1] };
 	const double dist = sqrt(dr[0]*dr[0] + dr[1]*dr[1]);
-	dr[0] /= dist; dr[1] /= dist;
+	const double idist = 1.0/dist;
+	dr[0] *= idist; dr[1] *= idist;
 	const double cl[4] = { wl[0], wl[1], wl[2], Tl }, cr[4] = { wr[0], wr[1], wr[2], Tr };
 	double grad[2][4];
 	for(int i = 0; i < 4; i++) {
@@ -58,11 +61,13 @@ __device__ __forceinline__ void viscous_flux(const GasParams& gp, const double* 
 			const double b = gl ? (i == 3 ? gtr[j] : gr[i*2+j]) : 0.0;
 			da[j] = 0.5*(a + b);
 		}
-		const double corr = (cr[i] - cl[i])/dist;
+		const double corr = (cr[i] - cl[i])*idist;
 		const double ddr = da[0]*dr[0] + da[1]*dr[1];
 		for(int j = 0; j < 2; j++) grad[j][i] = da[j] - ddr*dr[j] + corr*dr[j];
 	}
-	const double mu = gp.constvisc ? 1.0/gp.Reinf : 0.5*(viscosity(gp, ul) + viscosity(gp, ur));
+	double mu;
+	if(gp.constvisc) mu = 1.0/gp.Reinf;
+	else mu = 0.5*(sutherland(gp, A.p*A.ir*gp.gM2) + sutherland(gp, B.p*B.ir*gp.gM2));
 	const double kd = mu/(gp.Minf*gp.Minf*gp.gm1*gp.Pr);
 	const double ldiv = (grad[0][1] + grad[1][2])*(2.0/3.0*mu);
 	double s[2][2];
@@ -73,13 +78,19 @@ __device__ __forceinline__ void viscous_flux(const GasParams& gp, const double* 
 	vf[0] = 0.0;
 	vf[1] = -(s[0][0]*n[0] + s[0][1]*n[1]);
 	vf[2] = -(s[1][0]*n[0] + s[1][1]*n[1]);
-	const double va[2] = { 0.5*(ul[1]/ul[0] + ur[1]/ur[0]), 0.5*(ul[2]/ul[0] + ur[2]/ur[0]) };
+	const double va[2] = { 0.5*(A.vx + B.vx), 0.5*(A.vy + B.vy) };
 	double e = 0.0;
 	for(int i = 0; i < 2; i++) {
 		const double comp = s[i][0]*va[0] + s[i][1]*va[1] + kd*grad[i][3];
 		e -= comp*n[i];
 	}
 	vf[3] = e;
+}
+
+__device__ __forceinline__ void load_grad(const double* __restrict__ g, int c, double* v)
+{
+	load4(g + 8*c, v);
+	load4(g + 8*c + 4, v + 4);
 }
 
 template <int FLUX, int MODE, bool VISC>
@@ -97,6 +108,7 @@ void k_flux(int ncell, const double* __restrict__ u, const double* __restrict__ 
 	const int cc = valid ? cell : 0;
 	const int nb = nbr[cc*4 + q];
 	const bool act = valid && nb != SLOT_UNUSED;
+	const double igm1 = 1.0/gp.gm1;
 
 	double F[4] = {0.0, 0.0, 0.0, 0.0};
 	double srad = 0.0;
@@ -111,7 +123,7 @@ void k_flux(int ncell, const double* __restrict__ u, const double* __restrict__ 
 		double uL[4], wL[4], xL[2];
 		load4(u + 4*cL, uL);
 		xL[0] = centre[2*cL]; xL[1] = centre[2*cL+1];
-		to_primitive(gp, uL, wL);
+		const double irL = prim_lean(gp, uL, wL);
 
 		// right cell-centred state: neighbour cell, or ghost of the left cell value
 		double uR[4], wR[4], xR[2];
@@ -124,64 +136,72 @@ void k_flux(int ncell, const double* __restrict__ u, const double* __restrict__ 
 			ghost_state(gp, bt, bc, uL, n, uR);
 			xR[0] = 2.0*G.mx - xL[0]; xR[1] = 2.0*G.my - xL[1];
 		}
-		to_primitive(gp, uR, wR);
+		const double irR = prim_lean(gp, uR, wR);
 
-		double gL[8], gR[8];
-		double UL[4], UR[4];
+		FaceState A, B;
 		if constexpr(MODE == 0) {
-			for(int k = 0; k < 4; k++) UL[k] = uL[k];
-			if(!bnd) for(int k = 0; k < 4; k++) UR[k] = uR[k];
+			A = fs_both(gp, uL, wL, irL, n);
+			B = fs_both(gp, uR, wR, irR, n);
 		}
 		else if constexpr(MODE == 1) {
-			load4(rgrad + 8*cL, gL); load4(rgrad + 8*cL + 4, gL + 4);
-			double r[4];
+			double gL[8], r[4];
+			load_grad(rgrad, cL, gL);
 			const double dx = G.mx - xL[0], dy = G.my - xL[1];
 			for(int k = 0; k < 4; k++) r[k] = wL[k] + gL[2*k]*dx + gL[2*k+1]*dy;
-			to_conserved(gp, r, UL);
+			A = fs_prim(gp, r, n, igm1);
 			if(!bnd) {
-				load4(rgrad + 8*cR, gR); load4(rgrad + 8*cR + 4, gR + 4);
+				double gR[8];
+				load_grad(rgrad, cR, gR);
 				const double ex = G.mx - xR[0], ey = G.my - xR[1];
 				for(int k = 0; k < 4; k++) r[k] = wR[k] + gR[2*k]*ex + gR[2*k+1]*ey;
-				to_conserved(gp, r, UR);
+				B = fs_prim(gp, r, n, igm1);
 			}
 		}
 		else {
 			// MUSCL-Van Albada, kappa = 1/3, eps = 1e-8
-			load4(grad + 8*cL, gL); load4(grad + 8*cL + 4, gL + 4);
+			double gL[8], gR[8];
+			load_grad(grad, cL, gL);
+			if(!bnd) load_grad(grad, cR, gR);
 			const double ex = xR[0] - xL[0], ey = xR[1] - xL[1];
 			const double kap = 1.0/3.0, eps = 1e-8;
 			double rl[4], rr[4];
-			if(!bnd) { load4(grad + 8*cR, gR); load4(grad + 8*cR + 4, gR + 4); }
 			for(int k = 0; k < 4; k++) {
 				const double dw = wR[k] - wL[k];
 				const double dm = 2.0*(gL[2*k]*ex + gL[2*k+1]*ey) - dw;
 				double pl = (2.0*dm*dw + eps)/(dm*dm + dw*dw + eps);
 				if(pl < 0.0) pl = 0.0;
-				rl[k] = wL[k] + pl/4.0*((1.0 - kap*pl)*dm + (1.0 + kap*pl)*dw);
+				rl[k] = wL[k] + pl*0.25*((1.0 - kap*pl)*dm + (1.0 + kap*pl)*dw);
 				if(!bnd) {
 					const double dp = 2.0*(gR[2*k]*ex + gR[2*k+1]*ey) - dw;
 					double pr = (2.0*dp*dw + eps)/(dp*dp + dw*dw + eps);
 					if(pr < 0.0) pr = 0.0;
-					rr[k] = wR[k] - pr/4.0*((1.0 - kap*pr)*dp + (1.0 + kap*pr)*dw);
+					rr[k] = wR[k] - pr*0.25*((1.0 - kap*pr)*dp + (1.0 + kap*pr)*dw);
 				}
 			}
-			to_conserved(gp, rl, UL);
-			if(!bnd) to_conserved(gp, rr, UR);
+			A = fs_prim(gp, rl, n, igm1);
+			if(!bnd) B = fs_prim(gp, rr, n, igm1);
 		}
-		if(bnd) ghost_state(gp, bt, bc, UL, n, UR);
+		if constexpr(MODE != 0) {
+			if(bnd) {
+				const double UL[4] = { A.r, A.mx, A.my, A.E };
+				double UG[4];
+				ghost_state(gp, bt, bc, UL, n, UG);
+				B = fs_cons(gp, UG, n);
+			}
+		}
 
 		double FL[4];
-		num_flux<FLUX>(gp, UL, UR, n, FL);
+		fs_num_flux<FLUX>(gp, A, B, n, FL);
 		if constexpr(VISC) {
 			double vf[4];
 			if constexpr(MODE == 0) {
-				viscous_flux(gp, n, xL, xR, wL, wR, nullptr, nullptr, UL, UR, vf);
+				viscous_flux(gp, n, xL, xR, wL, irL, wR, irR, nullptr, nullptr, A, B, vf);
 			} else {
 				double vl[8], vr[8];
-				load4(grad + 8*cL, vl); load4(grad + 8*cL + 4, vl + 4);
-				if(!bnd) { load4(grad + 8*cR, vr); load4(grad + 8*cR + 4, vr + 4); }
+				load_grad(grad, cL, vl);
+				if(!bnd) load_grad(grad, cR, vr);
 				else for(int k = 0; k < 8; k++) vr[k] = vl[k];
-				viscous_flux(gp, n, xL, xR, wL, wR, vl, vr, UL, UR, vf);
+				viscous_flux(gp, n, xL, xR, wL, irL, wR, irR, vl, vr, A, B, vf);
 			}
 			for(int k = 0; k < 4; k++) FL[k] += vf[k];
 		}
@@ -189,14 +209,11 @@ void k_flux(int ncell, const double* __restrict__ u, const double* __restrict__ 
 		for(int k = 0; k < 4; k++) F[k] = sgn*FL[k];
 
 		if(getdt) {
-			const double* S = selfL ? UL : UR;
-			const double p = pressure(gp, S);
-			const double c = sqrt(gp.g*p/S[0]);
-			const double vn = (S[1]*n[0] + S[2]*n[1])/S[0];
-			srad = (fabs(vn) + c)*G.len;
+			const FaceState& S = selfL ? A : B;
+			srad = (fabs(S.vn) + S.c)*G.len;
 			if constexpr(VISC) {
-				const double mu = viscosity(gp, S);
-				const double co = fmax(4.0/(3*S[0]), gp.g/S[0]);
+				const double mu = gp.constvisc ? 1.0/gp.Reinf : sutherland(gp, S.p*S.ir*gp.gM2);
+				const double co = fmax(4.0/(3*S.r), gp.g/S.r);
 				srad += co*mu/gp.Pr*G.len*G.len/area[cc];
 			}
 		}

@@ -2,8 +2,8 @@
  *
  * Gradients (GradientScheme::compute_gradients, src/spatial/agradientschemes.cpp:61-440) are
  * evaluated as precomputed face stencils grad_i = sum_j c_ij (w_j - w_i) over the cell's
- * neighbours (ghost state of the cell value for a boundary face), one face per lane, summed
- * by a quad reduction. The Venkatakrishnan and Barth-Jespersen limiters
+ * neighbours (ghost state of the cell value for a boundary face), two lanes per cell each
+ * visiting two of its faces. The Venkatakrishnan and Barth-Jespersen limiters
  * (src/spatial/limitedlinearreconstruction.cpp:116-268) are fused into the same pass: the
  * neighbour differences that form the gradient also give the max/min bounds, and the limited
  * gradient phi*grad is what the flux pass reads. Boundary neighbours contribute the ghost
@@ -11,6 +11,7 @@
  */
 #include "fvx_internal.hpp"
 #include "wave.hpp"
+#include "face_state.hpp"
 
 namespace fvx {
 
@@ -25,6 +26,8 @@ FVX_DECL_FLUX(5) FVX_DECL_FLUX(6)
 /// LIM: 0 none, 1 Venkatakrishnan, 2 Barth-Jespersen. CONS: gradients of the conserved
 /// variables (getGradients) instead of the primitive ones. gout receives the unlimited
 /// gradient (may be null when LIM > 0 and it is not needed), rout the limited one.
+/// Two lanes per cell, each owning two of the cell's local faces; the pair combines its
+/// partial sums, bounds and limiter values with one DPP exchange.
 template <int LIM, bool CONS>
 __global__ __launch_bounds__(BLOCK)
 void k_gradients(int ncell, const double* __restrict__ u, const int* __restrict__ nbr,
@@ -34,77 +37,100 @@ void k_gradients(int ncell, const double* __restrict__ u, const int* __restrict_
                  const GasParams gp, const BCTable bt, double* __restrict__ gout,
                  double* __restrict__ rout)
 {
-	const int q = threadIdx.x & 3;
-	const int cell = xcd_block()*CELLS_PER_BLOCK + (threadIdx.x >> 2);
+	const int p = threadIdx.x & 1;
+	const int cell = xcd_block()*(BLOCK/2) + (threadIdx.x >> 1);
 	const bool valid = cell < ncell;
 	const int cc = valid ? cell : 0;
-	const int nb = nbr[cc*4 + q];
-	const bool act = valid && nb != SLOT_UNUSED;
-
 	double ui[4], wi[4];
 	load4(u + 4*cc, ui);
 	if constexpr(CONS) { for(int k = 0; k < 4; k++) wi[k] = ui[k]; }
-	else to_primitive(gp, ui, wi);
+	else prim_lean(gp, ui, wi);
+	const int2 nb2 = reinterpret_cast<const int2*>(nbr)[2*cc + p];
+	const int2 fc2 = reinterpret_cast<const int2*>(face)[2*cc + p];
+	const int nbs[2] = { valid ? nb2.x : SLOT_UNUSED, valid ? nb2.y : SLOT_UNUSED };
+	const int fcs[2] = { fc2.x, fc2.y };
+	const double2* cf = reinterpret_cast<const double2*>(gcoef) + 4*cc + 2*p;
 
-	double dw[4] = {0.0, 0.0, 0.0, 0.0};
-	double cx = 0.0, cy = 0.0;
-	FaceGeom G{};
-	if(act) {
-		G = fgeom[face[cc*4 + q]];
-		double uj[4], wj[4];
-		if(nb >= 0) load4(u + 4*nb, uj);
-		else {
-			const double n[2] = { G.nx, G.ny };
-			ghost_state(gp, bt, bcid[-1 - nb], ui, n, uj);
+	// both neighbour gathers in flight together (a boundary or unused slot loads the cell
+	// itself as a placeholder)
+	double uj[2][4];
+#pragma unroll
+	for(int k = 0; k < 2; k++) load4(u + 4*(nbs[k] >= 0 ? nbs[k] : cc), uj[k]);
+	const double2 c0 = cf[0], c1 = cf[1];
+	double fx[2] = {0, 0}, fy[2] = {0, 0};
+	if constexpr(LIM > 0) {
+#pragma unroll
+		for(int k = 0; k < 2; k++) {
+			if(nbs[k] == SLOT_UNUSED) continue;
+			const double2 m = *reinterpret_cast<const double2*>(&fgeom[fcs[k]].mx);
+			fx[k] = m.x; fy[k] = m.y;
 		}
-		if constexpr(CONS) { for(int k = 0; k < 4; k++) wj[k] = uj[k]; }
-		else to_primitive(gp, uj, wj);
-		for(int k = 0; k < 4; k++) dw[k] = wj[k] - wi[k];
-		const double2 c = reinterpret_cast<const double2*>(gcoef)[cc*4 + q];
-		cx = c.x; cy = c.y;
 	}
-	double g[8];
-	for(int k = 0; k < 4; k++) {
-		g[2*k] = quad_sum(cx*dw[k]);
-		g[2*k+1] = quad_sum(cy*dw[k]);
+#pragma unroll
+	for(int k = 0; k < 2; k++) {
+		const int nb = nbs[k];
+		if(nb < 0 && nb != SLOT_UNUSED) {
+			const FaceGeom G = fgeom[fcs[k]];
+			const double n[2] = { G.nx, G.ny };
+			ghost_state(gp, bt, bcid[-1 - nb], ui, n, uj[k]);
+		}
 	}
+
+	double g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+	double dmax[4] = {0, 0, 0, 0}, dmin[4] = {0, 0, 0, 0};
+#pragma unroll
+	for(int k = 0; k < 2; k++) {
+		if(nbs[k] == SLOT_UNUSED) continue;
+		const double2 c = k == 0 ? c0 : c1;
+		double wj[4];
+		if constexpr(CONS) { for(int v = 0; v < 4; v++) wj[v] = uj[k][v]; }
+		else prim_lean(gp, uj[k], wj);
+		for(int v = 0; v < 4; v++) {
+			const double dw = wj[v] - wi[v];
+			g[2*v] += c.x*dw;
+			g[2*v+1] += c.y*dw;
+			if constexpr(LIM > 0) { dmax[v] = fmax(dmax[v], dw); dmin[v] = fmin(dmin[v], dw); }
+		}
+	}
+	for(int k = 0; k < 8; k++) g[k] = pair_sum(g[k]);
 
 	if constexpr(LIM == 0) {
-		if(valid && q < 2) store4(gout + 8*cc + 4*q, g + 4*q);
+		if(valid) store4(gout + 8*cc + 4*p, g + 4*p);
 	} else {
-		double phi[4];
 		const double xi = centre[2*cc], yi = centre[2*cc+1];
 		const double e2 = LIM == 1 ? eps2[cc] : 0.0;
-		for(int k = 0; k < 4; k++) {
-			const double dmax = quad_max(act ? fmax(dw[k], 0.0) : 0.0);
-			const double dmin = quad_min(act ? fmin(dw[k], 0.0) : 0.0);
-			double ph = 1.0;
-			if(act) {
+		double lg[8];
+		for(int v = 0; v < 4; v++) {
+			const double mx = pair_max(dmax[v]), mn = pair_min(dmin[v]);
+			double phi = 1.0;
+			for(int k = 0; k < 2; k++) {
+				if(nbs[k] == SLOT_UNUSED) continue;
 				// face value by linear extrapolation, then its difference from the cell value
-				double uf = wi[k];
-				uf += g[2*k]*(G.mx - xi);
-				uf += g[2*k+1]*(G.my - yi);
-				const double dm = uf - wi[k];
+				double uf = wi[v];
+				uf += g[2*v]*(fx[k] - xi);
+				uf += g[2*v+1]*(fy[k] - yi);
+				const double dm = uf - wi[v];
+				double ph;
 				if constexpr(LIM == 1) {
-					const double dp = dm < 0 ? dmin : dmax;
+					const double dp = dm < 0 ? mn : mx;
 					ph = (dp*dp + 2*dp*dm + e2)/(dp*dp + dp*dm + 2*dm*dm + e2);
 				} else {
-					if(dm > 0) ph = 1 < dmax/dm ? 1.0 : dmax/dm;
-					else if(dm < 0) ph = 1 < dmin/dm ? 1.0 : dmin/dm;
+					if(dm > 0) ph = 1 < mx/dm ? 1.0 : mx/dm;
+					else if(dm < 0) ph = 1 < mn/dm ? 1.0 : mn/dm;
+					else ph = 1.0;
 				}
+				phi = fmin(phi, ph);
 			}
-			phi[k] = fmin(quad_min(ph), 1.0);
+			phi = pair_min(phi);
+			lg[2*v] = phi*g[2*v];
+			lg[2*v+1] = phi*g[2*v+1];
 		}
-		double lg[8];
-		for(int k = 0; k < 4; k++) { lg[2*k] = phi[k]*g[2*k]; lg[2*k+1] = phi[k]*g[2*k+1]; }
 		if(valid) {
-			if(q < 2) { if(gout) store4(gout + 8*cc + 4*q, g + 4*q); }
-			else store4(rout + 8*cc + 4*(q-2), lg + 4*(q-2));
+			if(gout) store4(gout + 8*cc + 4*p, g + 4*p);
+			store4(rout + 8*cc + 4*p, lg + 4*p);
 		}
 	}
 }
-
-static int grid_of(int ncell) { return (ncell + CELLS_PER_BLOCK - 1)/CELLS_PER_BLOCK; }
 
 /* ------------------------------------------------------------------ profiler ----------- */
 
@@ -157,7 +183,7 @@ void launch_gradients(const fvx_disc_s& d, const double* u, double* grad, double
                       int conserved, hipStream_t s)
 {
 	const fvx_mesh_s& m = *d.m;
-	const int nblk = grid_of(m.nelem);
+	const int nblk = (m.nelem + BLOCK/2 - 1)/(BLOCK/2);
 	const int lim = conserved ? 0 : (d.nc.reconstruction == FVX_RECON_VENKATAKRISHNAN ? 1 :
 	                                 d.nc.reconstruction == FVX_RECON_BARTHJESPERSEN ? 2 : 0);
 #define FVX_GL(L, C) hipLaunchKernelGGL((k_gradients<L, C>), dim3(nblk), dim3(BLOCK), 0, s, \

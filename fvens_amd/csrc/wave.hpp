@@ -47,6 +47,11 @@ __device__ __forceinline__ double quad_min(double v)
 	return fmin(v, dpp_d<DPP_XOR2>(v));
 }
 
+/// Sum, maximum and minimum over the two lanes of a pair (lane^1).
+__device__ __forceinline__ double pair_sum(double v) { return v + dpp_d<DPP_XOR1>(v); }
+__device__ __forceinline__ double pair_max(double v) { return fmax(v, dpp_d<DPP_XOR1>(v)); }
+__device__ __forceinline__ double pair_min(double v) { return fmin(v, dpp_d<DPP_XOR1>(v)); }
+
 /// Logical workgroup index such that each XCD (workgroups are dealt round-robin over the 8
 /// XCDs) processes one contiguous range of cells: neighbours gathered by one workgroup are
 /// then mostly resident in the same XCD's L2.

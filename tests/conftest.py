@@ -10,6 +10,13 @@ for p in (ROOT, os.path.join(ROOT, "oracle")):
 
 REF_INPUT = os.path.join(ROOT, "tests", "golden")
 
+# the first torch import on a fresh machine can take minutes; do it at collection time, outside
+# the per-test time limits
+try:
+    import torch  # noqa: F401
+except ImportError:
+    pass
+
 
 def pytest_configure(config):
     config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP library)")
