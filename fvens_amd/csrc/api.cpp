@@ -126,6 +126,152 @@ fvx_status fvx_get_primitive_gradients(fvx_disc d, const double* u, double* grad
 	});
 }
 
+/* ------------------------------------------------------------- implicit solve ---------- */
+
+fvx_status fvx_matrix_create(fvx_mesh m, int precision, fvx_matrix* out)
+{
+	return guarded([&] {
+		require(m && out, "null argument");
+		std::unique_ptr<fvx_matrix_s> A(new fvx_matrix_s);
+		fvx::matrix_init(*A, m, precision);
+		*out = A.release();
+	});
+}
+
+fvx_status fvx_matrix_destroy(fvx_matrix A)
+{
+	return guarded([&] { delete A; });
+}
+
+fvx_status fvx_assemble_jacobian(fvx_disc d, const double* u, fvx_matrix A, void* stream)
+{
+	return guarded([&] {
+		require(d && u && A, "null argument");
+		require(A->m == d->m, "matrix and discretisation belong to different meshes");
+		if(!fvx::jacobian_supported(*d))
+			throw fvx::Error(FVX_ERR_UNSUPPORTED, "the reference has no Jacobian for this flux or BC");
+		fvx::launch_jacobian(*d, u, A->raw(), A->prec, nullptr, 0.0, nullptr,
+		                     static_cast<hipStream_t>(stream));
+	});
+}
+
+fvx_status fvx_add_pseudotime(fvx_disc d, fvx_matrix A, const double* dt, double cfl,
+                              double* dtm, void* stream)
+{
+	return guarded([&] {
+		require(d && A && dt && dtm, "null argument");
+		require(A->m == d->m, "matrix and discretisation belong to different meshes");
+		require(cfl > 0, "CFL number must be positive");
+		fvx::launch_pseudotime(*A, dt, cfl, dtm, static_cast<hipStream_t>(stream));
+	});
+}
+
+fvx_status fvx_matrix_apply(fvx_matrix A, const double* x, double* y, void* stream)
+{
+	return guarded([&] {
+		require(A && x && y, "null argument");
+		fvx::launch_spmv(*A, x, y, static_cast<hipStream_t>(stream));
+	});
+}
+
+fvx_status fvx_matrix_bsr_nnzb(fvx_matrix A, int* nnzb)
+{
+	return guarded([&] {
+		require(A && nnzb, "null argument");
+		*nnzb = fvx::matrix_bsr_nnzb(*A->m);
+	});
+}
+
+fvx_status fvx_matrix_export_bsr(fvx_matrix A, int* rowptr, int* colind, double* vals)
+{
+	return guarded([&] {
+		require(A != nullptr, "null handle");
+		fvx::matrix_export_bsr(*A, rowptr, colind, vals);
+	});
+}
+
+fvx_status fvx_pc_create(fvx_mesh m, int type, int precision, fvx_pc* out)
+{
+	return guarded([&] {
+		require(m && out, "null argument");
+		std::unique_ptr<fvx_pc_s> p(new fvx_pc_s);
+		fvx::pc_init(*p, m, type, precision);
+		*out = p.release();
+	});
+}
+
+fvx_status fvx_pc_destroy(fvx_pc p)
+{
+	return guarded([&] { delete p; });
+}
+
+fvx_status fvx_pc_setup(fvx_pc p, fvx_matrix A, void* stream)
+{
+	return guarded([&] {
+		require(p && A, "null argument");
+		require(p->m == A->m, "preconditioner and matrix belong to different meshes");
+		fvx::launch_pc_setup(*p, *A, static_cast<hipStream_t>(stream));
+	});
+}
+
+fvx_status fvx_pc_apply(fvx_pc p, const double* r, double* z, void* stream)
+{
+	return guarded([&] {
+		require(p && r && z, "null argument");
+		fvx::launch_pc_apply(*p, r, z, static_cast<hipStream_t>(stream));
+	});
+}
+
+fvx_status fvx_pc_ordering(fvx_pc p, int* colour, int* ncolours)
+{
+	return guarded([&] {
+		require(p != nullptr, "null handle");
+		if(colour) std::memcpy(colour, p->colour.data(), sizeof(int)*p->colour.size());
+		if(ncolours) *ncolours = p->ncol;
+	});
+}
+
+fvx_status fvx_linear_solve(fvx_matrix A, fvx_pc p, const double* b, double* x, int restart,
+                            double rtol, int* iters, void* stream)
+{
+	return guarded([&] {
+		require(A && p && b && x, "null argument");
+		require(p->m == A->m, "preconditioner and matrix belong to different meshes");
+		const int it = fvx::linear_solve(*A, *p, b, x, restart, rtol,
+		                                 static_cast<hipStream_t>(stream));
+		if(iters) *iters = it;
+	});
+}
+
+fvx_status fvx_matfree_apply(fvx_disc d, const double* u, const double* r_u, const double* dtm,
+                             const double* x, double* y, double eps, void* stream)
+{
+	return guarded([&] {
+		require(d && u && r_u && dtm && x && y, "null argument");
+		require(eps > 0, "difference step must be positive");
+		hipStream_t s = static_cast<hipStream_t>(stream);
+		const size_t N = 4*(size_t)d->m->nelem;
+		fvx::DBuf<double> aux, raux, part;
+		aux.alloc(N);
+		raux.alloc(N);
+		part.alloc((size_t)fvx::reduction_blocks()*fvx::max_dots() + 1);
+		fvx::matfree_apply(*d, u, r_u, dtm, x, y, eps, aux.p, raux.p, part.p,
+		                   part.p + (size_t)fvx::reduction_blocks()*fvx::max_dots(), s);
+		FVX_HIP(hipStreamSynchronize(s));
+	});
+}
+
+fvx_status fvx_steady_solve(fvx_disc d, const fvx_solver_cfg* cfg, double* u,
+                            fvx_solve_stats* stats, void* stream)
+{
+	fvx_solve_stats local;
+	fvx_solve_stats* st = stats ? stats : &local;
+	return guarded([&] {
+		require(d && cfg && u, "null argument");
+		fvx::steady_solve(*d, *cfg, u, *st, static_cast<hipStream_t>(stream));
+	});
+}
+
 fvx_status fvx_profile_enable(fvx_disc d, int on)
 {
 	return guarded([&] {

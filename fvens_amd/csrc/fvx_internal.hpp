@@ -94,6 +94,8 @@ struct Profiler {
 	~Profiler();
 };
 enum { PROF_GRADIENTS = 0, PROF_FLUX = 1, PROF_JACOBIAN = 2, PROF_PC = 3 };
+struct SolverWork;
+void destroy_solver_work(SolverWork* w);
 }
 
 /// Discretisation handle: configuration plus per-mesh coefficient arrays and scratch.
@@ -111,6 +113,35 @@ struct fvx_disc_s {
 	fvx::DBuf<double> d_grad;              // [nelem][8] unlimited primitive gradients
 	fvx::DBuf<double> d_rgrad;             // [nelem][8] reconstruction gradients (limited)
 	fvx::DBuf<double> d_scratch;           // reductions for functionals
+	fvx::SolverWork* work = nullptr;       // implicit-solver workspace, created on first use
+	~fvx_disc_s() { fvx::destroy_solver_work(work); }
+};
+
+/// Block matrix on the cell graph: vals[cell][slot][4][4], slot 0 the diagonal block and slot
+/// 1+k the block of neighbour nbr[cell][k] (zero where there is none).
+struct fvx_matrix_s {
+	const fvx_mesh_s* m = nullptr;
+	int prec = FVX_PREC_F64;
+	fvx::DBuf<double> v64;
+	fvx::DBuf<float> v32;
+	double* f64() const { return v64.p; }
+	float* f32() const { return v32.p; }
+	void* raw() const { return prec == FVX_PREC_F32 ? (void*)v32.p : (void*)v64.p; }
+};
+
+/// Preconditioner: factors in the matrix layout with slot 0 holding the inverted diagonal
+/// block; multicolour ordering (colour per cell, cells listed by colour, lower-colour mask).
+struct fvx_pc_s {
+	const fvx_mesh_s* m = nullptr;
+	int type = FVX_PC_NONE, prec = FVX_PREC_F64;
+	int ncol = 0;
+	std::vector<int> colour, order, coff;
+	fvx::DBuf<int> d_colour, d_order;
+	fvx::DBuf<unsigned char> d_lowmask;
+	fvx::DBuf<double> v64;
+	fvx::DBuf<float> v32;
+	double* f64() const { return v64.p; }
+	float* f32() const { return v32.p; }
 };
 
 namespace fvx {
@@ -126,6 +157,52 @@ void launch_residual(const fvx_disc_s& d, const double* u, double* r, int gettim
 void launch_functionals(const fvx_disc_s& d, const double* u, const double* cgrad, int walltag,
                         double* dev_out, hipStream_t s);
 int functional_blocks();
+
+// Jacobian assembly (jacobian.hip)
+bool jacobian_supported(const fvx_disc_s& d);
+void launch_jacobian(const fvx_disc_s& d, const double* u, void* vals, int prec,
+                     const double* dt, double cfl, double* dtm, hipStream_t s);
+
+// matrix, preconditioner and vector kernels (linalg.hip)
+void launch_spmv(const fvx_matrix_s& A, const double* x, double* y, hipStream_t s);
+void launch_pseudotime(fvx_matrix_s& A, const double* dt, double cfl, double* dtm,
+                       hipStream_t s);
+void launch_pc_setup(fvx_pc_s& p, const fvx_matrix_s& A, hipStream_t s);
+void launch_pc_apply(const fvx_pc_s& p, const double* r, double* z, hipStream_t s);
+int reduction_blocks();
+int max_dots();
+void launch_mdot(long N, const double* V, long ldv, int nv, const double* w, double* partial,
+                 double* out, hipStream_t s);
+void launch_gs_update(long N, const double* V, long ldv, int nv, const double* h, double* w,
+                      double* partial, double* nrm2, hipStream_t s);
+void launch_normalise(long N, double* w, const double* nrm2, hipStream_t s);
+void launch_scale_copy(long N, const double* b, const double* nrm2, double* out, hipStream_t s);
+void launch_lincomb(long N, const double* Z, long ldz, int nv, const double* y, double* x,
+                    hipStream_t s);
+void launch_resnorm(int n, const double* r, const double* area, double* partial, double* out,
+                    hipStream_t s);
+void launch_mf_prep(long N, const double* u, const double* x, const double* xx, double eps,
+                    double* aux, hipStream_t s);
+void launch_mf_finish(int n, const double* x, const double* dtm, const double* ru,
+                      const double* raux, const double* xx, double eps, double* y,
+                      hipStream_t s);
+void launch_update_implicit(int n, double* u, const double* du, int full, double min_relax,
+                            double gamma, hipStream_t s);
+void launch_update_explicit(int n, double* u, const double* r, const double* dt,
+                            const double* area, double cfl, hipStream_t s);
+
+// host side of the implicit solver (solver.cpp)
+void matrix_init(fvx_matrix_s& A, const fvx_mesh_s* m, int prec);
+void pc_init(fvx_pc_s& p, const fvx_mesh_s* m, int type, int prec);
+void matrix_export_bsr(const fvx_matrix_s& A, int* rowptr, int* colind, double* vals);
+int matrix_bsr_nnzb(const fvx_mesh_s& m);
+void matfree_apply(const fvx_disc_s& d, const double* u, const double* ru, const double* dtm,
+                   const double* x, double* y, double eps, double* aux, double* raux,
+                   double* partial, double* xx, hipStream_t s);
+int linear_solve(const fvx_matrix_s& A, const fvx_pc_s& p, const double* b, double* x,
+                 int restart, double rtol, hipStream_t s);
+void steady_solve(fvx_disc_s& d, const fvx_solver_cfg& cfg, double* u, fvx_solve_stats& st,
+                  hipStream_t s);
 }
 
 #endif

@@ -1,0 +1,747 @@
+/* linalg.hip -- kernels of the implicit solver: block-sparse matrix, preconditioners and the
+ * vector operations of the Krylov solver.
+ *
+ * Matrix storage ("slot blocks", shared with jacobian.hip): vals[cell][slot][4][4] row-major,
+ * slot 0 the diagonal block, slot 1+k the block coupling the cell to nbr[cell][k]; zero where
+ * that slot has no interior neighbour. A preconditioner stores its factors in the same layout
+ * with slot 0 holding the INVERTED diagonal block.
+ *
+ * Multicolour sweeps: cells are coloured so that no two cells of one colour share a face, and
+ * the preconditioner sweeps colour by colour. Within a colour every row depends only on rows of
+ * other colours, so one launch processes a whole colour; the result equals the sequential
+ * sweep over the cells sorted by (colour, index). lowmask[cell] bit k marks neighbour slot k as
+ * belonging to a lower colour.
+ *
+ * Reductions (dot products, norms) are deterministic: fixed grid, fixed per-thread order, fixed
+ * tree within the block, then one block sums the per-block partials in order.
+ */
+#include "fvx_internal.hpp"
+#include "wave.hpp"
+
+namespace fvx {
+
+/* ----------------------------------------------------------------- block helpers ------- */
+
+template <class VT> __device__ __forceinline__ void load_row(const VT* p, double* v);
+template <> __device__ __forceinline__ void load_row<double>(const double* p, double* v)
+{
+	const double2 a = reinterpret_cast<const double2*>(p)[0];
+	const double2 b = reinterpret_cast<const double2*>(p)[1];
+	v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+}
+template <> __device__ __forceinline__ void load_row<float>(const float* p, double* v)
+{
+	const float4 a = *reinterpret_cast<const float4*>(p);
+	v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+}
+template <class VT> __device__ __forceinline__ void put_row(VT* p, const double* v);
+template <> __device__ __forceinline__ void put_row<double>(double* p, const double* v)
+{
+	reinterpret_cast<double2*>(p)[0] = make_double2(v[0], v[1]);
+	reinterpret_cast<double2*>(p)[1] = make_double2(v[2], v[3]);
+}
+template <> __device__ __forceinline__ void put_row<float>(float* p, const double* v)
+{
+	*reinterpret_cast<float4*>(p) = make_float4((float)v[0], (float)v[1], (float)v[2],
+	                                            (float)v[3]);
+}
+template <class VT> __device__ __forceinline__ void load_block(const VT* p, double* b)
+{
+	for(int r = 0; r < 4; r++) load_row<VT>(p + 4*r, b + 4*r);
+}
+template <class VT> __device__ __forceinline__ void put_block(VT* p, const double* b)
+{
+	for(int r = 0; r < 4; r++) put_row<VT>(p + 4*r, b + 4*r);
+}
+
+__device__ __forceinline__ double dot4(const double* a, const double* x)
+{
+	return a[0]*x[0] + a[1]*x[1] + a[2]*x[2] + a[3]*x[3];
+}
+
+/// C = A B for 4x4 row-major blocks.
+__device__ __forceinline__ void gemm4(const double* A, const double* B, double* C)
+{
+	for(int i = 0; i < 4; i++)
+		for(int j = 0; j < 4; j++)
+			C[i*4+j] = A[i*4]*B[j] + A[i*4+1]*B[4+j] + A[i*4+2]*B[8+j] + A[i*4+3]*B[12+j];
+}
+
+/// Inverse of a 4x4 block by Gauss-Jordan elimination with partial pivoting; row exchanges
+/// are done with selects so that every index is static (no register-array spills).
+__device__ __forceinline__ void invert4(const double* a, double* inv)
+{
+	double m[4][8];
+	for(int i = 0; i < 4; i++)
+		for(int j = 0; j < 8; j++) m[i][j] = j < 4 ? a[i*4+j] : (j - 4 == i ? 1.0 : 0.0);
+	for(int c = 0; c < 4; c++) {
+		int p = c;
+		double best = fabs(m[c][c]);
+		for(int r = c + 1; r < 4; r++)
+			if(fabs(m[r][c]) > best) { best = fabs(m[r][c]); p = r; }
+		for(int r = c + 1; r < 4; r++) {
+			const bool sw = p == r;
+			for(int j = 0; j < 8; j++) {
+				const double x = m[c][j], y = m[r][j];
+				m[c][j] = sw ? y : x;
+				m[r][j] = sw ? x : y;
+			}
+		}
+		const double piv = m[c][c];
+		for(int j = 0; j < 8; j++) m[c][j] /= piv;
+		for(int r = 0; r < 4; r++) {
+			if(r == c) continue;
+			const double f = m[r][c];
+			for(int j = 0; j < 8; j++) m[r][j] -= f*m[c][j];
+		}
+	}
+	for(int i = 0; i < 4; i++)
+		for(int j = 0; j < 4; j++) inv[i*4+j] = m[i][j+4];
+}
+
+/// Value of lane k of the quad, on every lane of the quad.
+template <int K> __device__ __forceinline__ double quad_bcast(double v)
+{
+	return dpp_d<K*0x55>(v);
+}
+
+__device__ __forceinline__ int4 load_nbr(const int* nbr, int c)
+{
+	return reinterpret_cast<const int4*>(nbr)[c];
+}
+
+/* ------------------------------------------------------------------ matrix ------------- */
+
+/// y = A x; one quad per cell, lane q computes row q.
+template <class VT>
+__global__ __launch_bounds__(BLOCK)
+void k_spmv(int n, const VT* __restrict__ vals, const int* __restrict__ nbr,
+            const double* __restrict__ x, double* __restrict__ y)
+{
+	const int q = threadIdx.x & 3;
+	const int c = xcd_block()*CELLS_PER_BLOCK + (threadIdx.x >> 2);
+	if(c >= n) return;
+	const int4 nb = load_nbr(nbr, c);
+	const int nbs[4] = { nb.x, nb.y, nb.z, nb.w };
+	const VT* blk = vals + (size_t)c*80 + 4*q;
+	double a[4], xv[4];
+	load_row<VT>(blk, a);
+	load4(x + 4*c, xv);
+	double acc = dot4(a, xv);
+	for(int k = 0; k < 4; k++) {
+		if(nbs[k] < 0) continue;
+		load_row<VT>(blk + 16*(1 + k), a);
+		load4(x + 4*nbs[k], xv);
+		acc += dot4(a, xv);
+	}
+	y[4*c + q] = acc;
+}
+
+/// A_ii += dtm_i I with dtm_i = area_i/(cfl dt_i).
+template <class VT>
+__global__ __launch_bounds__(BLOCK)
+void k_pseudotime(int n, VT* __restrict__ vals, const double* __restrict__ dt,
+                  const double* __restrict__ area, double cfl, double* __restrict__ dtm)
+{
+	const int q = threadIdx.x & 3;
+	const int c = xcd_block()*CELLS_PER_BLOCK + (threadIdx.x >> 2);
+	if(c >= n) return;
+	const double add = area[c]/(cfl*dt[c]);
+	if(q == 0) dtm[c] = add;
+	VT* p = vals + (size_t)c*80 + 5*q;
+	*p = (VT)((double)*p + add);
+}
+
+/* ---------------------------------------------------------- preconditioners ------------ */
+
+/// Inverts the diagonal block of every cell into fac slot 0; with copy_off, also copies the
+/// off-diagonal blocks (symmetric Gauss-Seidel sweeps read them from the factor array).
+template <class VTA, class VTF>
+__global__ __launch_bounds__(BLOCK)
+void k_invert_diag(int n, const VTA* __restrict__ vals, VTF* __restrict__ fac, int copy_off)
+{
+	const int c = blockIdx.x*blockDim.x + threadIdx.x;
+	if(c >= n) return;
+	double a[16], inv[16];
+	load_block<VTA>(vals + (size_t)c*80, a);
+	invert4(a, inv);
+	put_block<VTF>(fac + (size_t)c*80, inv);
+	if(copy_off)
+		for(int s = 1; s < 5; s++) {
+			load_block<VTA>(vals + (size_t)c*80 + 16*s, a);
+			put_block<VTF>(fac + (size_t)c*80 + 16*s, a);
+		}
+}
+
+/// z = D^-1 r, one quad per cell.
+template <class VTF>
+__global__ __launch_bounds__(BLOCK)
+void k_jacobi_apply(int n, const VTF* __restrict__ fac, const double* __restrict__ r,
+                    double* __restrict__ z)
+{
+	const int q = threadIdx.x & 3;
+	const int c = xcd_block()*CELLS_PER_BLOCK + (threadIdx.x >> 2);
+	if(c >= n) return;
+	double a[4], rv[4];
+	load_row<VTF>(fac + (size_t)c*80 + 4*q, a);
+	load4(r + 4*c, rv);
+	z[4*c + q] = dot4(a, rv);
+}
+
+/// One colour of a symmetric block Gauss-Seidel sweep (PETSc SOR with omega = 1, one local
+/// symmetric iteration from zero). Forward: z_i = D_i^-1 (r_i - sum_{lower j} A_ij z_j);
+/// backward: z_i = D_i^-1 (r_i - sum_{j != i} A_ij z_j).
+template <class VTF, bool BACK>
+__global__ __launch_bounds__(BLOCK)
+void k_sgs_colour(int cnt, const int* __restrict__ list, const VTF* __restrict__ fac,
+                  const int* __restrict__ nbr, const unsigned char* __restrict__ lowmask,
+                  const double* __restrict__ r, double* __restrict__ z)
+{
+	const int q = threadIdx.x & 3;
+	const int idx = xcd_block()*CELLS_PER_BLOCK + (threadIdx.x >> 2);
+	const bool valid = idx < cnt;
+	const int i = valid ? list[idx] : list[0];
+	const int4 nb = load_nbr(nbr, i);
+	const int nbs[4] = { nb.x, nb.y, nb.z, nb.w };
+	const int lm = lowmask[i];
+	const VTF* blk = fac + (size_t)i*80 + 4*q;
+	double t = r[4*i + q];
+	double a[4], zv[4];
+	for(int k = 0; k < 4; k++) {
+		if(nbs[k] < 0 || !(BACK || ((lm >> k) & 1))) continue;
+		load_row<VTF>(blk + 16*(1 + k), a);
+		load4(z + 4*nbs[k], zv);
+		t -= dot4(a, zv);
+	}
+	const double tv[4] = { quad_bcast<0>(t), quad_bcast<1>(t), quad_bcast<2>(t),
+	                       quad_bcast<3>(t) };
+	load_row<VTF>(blk, a);
+	if(valid) z[4*i + q] = dot4(a, tv);
+}
+
+/// Forward substitution of block ILU(0) for one colour: y_i = r_i - sum_{lower k} L_ik y_k.
+template <class VTF>
+__global__ __launch_bounds__(BLOCK)
+void k_ilu_fwd(int cnt, const int* __restrict__ list, const VTF* __restrict__ fac,
+               const int* __restrict__ nbr, const unsigned char* __restrict__ lowmask,
+               const double* __restrict__ r, double* __restrict__ z)
+{
+	const int q = threadIdx.x & 3;
+	const int idx = xcd_block()*CELLS_PER_BLOCK + (threadIdx.x >> 2);
+	if(idx >= cnt) return;
+	const int i = list[idx];
+	const int4 nb = load_nbr(nbr, i);
+	const int nbs[4] = { nb.x, nb.y, nb.z, nb.w };
+	const int lm = lowmask[i];
+	const VTF* blk = fac + (size_t)i*80 + 4*q;
+	double t = r[4*i + q];
+	double a[4], zv[4];
+	for(int k = 0; k < 4; k++) {
+		if(!((lm >> k) & 1)) continue;
+		load_row<VTF>(blk + 16*(1 + k), a);
+		load4(z + 4*nbs[k], zv);
+		t -= dot4(a, zv);
+	}
+	z[4*i + q] = t;
+}
+
+/// Backward substitution of block ILU(0) for one colour, in place on z (which holds y):
+/// z_i = D_i^-1 (y_i - sum_{upper j} U_ij z_j).
+template <class VTF>
+__global__ __launch_bounds__(BLOCK)
+void k_ilu_bwd(int cnt, const int* __restrict__ list, const VTF* __restrict__ fac,
+               const int* __restrict__ nbr, const unsigned char* __restrict__ lowmask,
+               double* __restrict__ z)
+{
+	const int q = threadIdx.x & 3;
+	const int idx = xcd_block()*CELLS_PER_BLOCK + (threadIdx.x >> 2);
+	const bool valid = idx < cnt;
+	const int i = valid ? list[idx] : list[0];
+	const int4 nb = load_nbr(nbr, i);
+	const int nbs[4] = { nb.x, nb.y, nb.z, nb.w };
+	const int lm = lowmask[i];
+	const VTF* blk = fac + (size_t)i*80 + 4*q;
+	double t = z[4*i + q];
+	double a[4], zv[4];
+	for(int k = 0; k < 4; k++) {
+		if(nbs[k] < 0 || ((lm >> k) & 1)) continue;
+		load_row<VTF>(blk + 16*(1 + k), a);
+		load4(z + 4*nbs[k], zv);
+		t -= dot4(a, zv);
+	}
+	const double tv[4] = { quad_bcast<0>(t), quad_bcast<1>(t), quad_bcast<2>(t),
+	                       quad_bcast<3>(t) };
+	load_row<VTF>(blk, a);
+	if(valid) z[4*i + q] = dot4(a, tv);
+}
+
+/// Block ILU(0) factorisation of the rows of one colour (IKJ form on the matrix pattern,
+/// rows taken in (colour, index) order; one thread per row). For each lower neighbour k in
+/// ascending order: L_ik = A_ik U_kk^-1, then A_ij -= L_ik U_kj for every j > k in the
+/// pattern of both rows (the diagonal j = i always; other j only where three cells are
+/// mutually adjacent). Finally slot 0 receives inv(A_ii).
+template <class VTA, class VTF>
+__global__ __launch_bounds__(BLOCK)
+void k_ilu_factor(int cnt, const int* __restrict__ list, const VTA* __restrict__ vals,
+                  VTF* __restrict__ fac, const int* __restrict__ nbr,
+                  const unsigned char* __restrict__ lowmask, const int* __restrict__ colour,
+                  int n)
+{
+	const int idx = blockIdx.x*blockDim.x + threadIdx.x;
+	if(idx >= cnt) return;
+	const int i = list[idx];
+	const int4 nb4 = load_nbr(nbr, i);
+	const int nb[4] = { nb4.x, nb4.y, nb4.z, nb4.w };
+	const int lm = lowmask[i];
+	double A[5][16];
+	for(int s = 0; s < 5; s++) load_block<VTA>(vals + (size_t)i*80 + 16*s, A[s]);
+	// keys (colour, index) of the neighbours; lower slots sorted ascending by key
+	long long key[4];
+	for(int k = 0; k < 4; k++)
+		key[k] = nb[k] >= 0 ? (long long)colour[nb[k]]*n + nb[k] : -1;
+	const long long BIG = 0x7fffffffffffffffLL;
+	long long sk[4];
+	int ss[4];
+	for(int k = 0; k < 4; k++) { sk[k] = ((lm >> k) & 1) ? key[k] : BIG; ss[k] = k; }
+	// sorting network on 4 (key, slot) pairs
+#define FVX_CSWAP(a, b) { const bool sw = sk[b] < sk[a]; \
+		const long long tk = sw ? sk[b] : sk[a]; const long long uk = sw ? sk[a] : sk[b]; \
+		const int ts = sw ? ss[b] : ss[a]; const int us = sw ? ss[a] : ss[b]; \
+		sk[a] = tk; sk[b] = uk; ss[a] = ts; ss[b] = us; }
+	FVX_CSWAP(0, 1) FVX_CSWAP(2, 3) FVX_CSWAP(0, 2) FVX_CSWAP(1, 3) FVX_CSWAP(1, 2)
+#undef FVX_CSWAP
+	for(int t = 0; t < 4; t++) {
+		if(sk[t] == BIG) break;
+		const int k = ss[t];
+		const int kc = nb[k];
+		const long long kkey = sk[t];
+		double Dk[16], Ak[16], L[16];
+		load_block<VTF>(fac + (size_t)kc*80, Dk);
+		for(int e = 0; e < 16; e++) {
+			double v = A[1][e];
+			for(int s = 1; s < 4; s++) v = k == s ? A[1+s][e] : v;
+			Ak[e] = v;
+		}
+		gemm4(Ak, Dk, L);
+		for(int s = 0; s < 4; s++)
+			if(k == s) for(int e = 0; e < 16; e++) A[1+s][e] = L[e];
+		const int4 kn4 = load_nbr(nbr, kc);
+		const int kn[4] = { kn4.x, kn4.y, kn4.z, kn4.w };
+		// j = i: the diagonal
+		{
+			int ti = 0;
+			for(int s = 1; s < 4; s++) if(kn[s] == i) ti = s;
+			double U[16], P[16];
+			load_block<VTF>(fac + (size_t)kc*80 + 16*(1 + ti), U);
+			gemm4(L, U, P);
+			for(int e = 0; e < 16; e++) A[0][e] -= P[e];
+		}
+		// j = another neighbour of i that follows k and is also a neighbour of k
+		for(int s = 0; s < 4; s++) {
+			const int j = nb[s];
+			if(s == k || j < 0 || key[s] <= kkey) continue;
+			int tj = -1;
+			for(int r = 0; r < 4; r++) if(kn[r] == j) tj = r;
+			if(tj < 0) continue;
+			double U[16], P[16];
+			load_block<VTF>(fac + (size_t)kc*80 + 16*(1 + tj), U);
+			gemm4(L, U, P);
+			for(int e = 0; e < 16; e++) A[1+s][e] -= P[e];
+		}
+	}
+	double inv[16];
+	invert4(A[0], inv);
+	put_block<VTF>(fac + (size_t)i*80, inv);
+	for(int s = 1; s < 5; s++) put_block<VTF>(fac + (size_t)i*80 + 16*s, A[s]);
+}
+
+/* ------------------------------------------------------------- reductions --------------- */
+
+constexpr int RED_BLOCKS = 512;
+constexpr int MAX_DOTS = 32;
+
+/// Sum over the 256 threads of a block; the result is valid in thread 0.
+__device__ __forceinline__ double block_sum(double v, double* sh)
+{
+	for(int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+	const int w = threadIdx.x >> 6;
+	__syncthreads();
+	if((threadIdx.x & 63) == 0) sh[w] = v;
+	__syncthreads();
+	return (sh[0] + sh[1]) + (sh[2] + sh[3]);
+}
+
+/// partial[b][i] = block b's part of dot(V_i, w), i < nv (V_i = V + i*ldv).
+__global__ __launch_bounds__(BLOCK)
+void k_mdot(long N, const double* __restrict__ V, long ldv, int nv,
+            const double* __restrict__ w, double* __restrict__ partial)
+{
+	__shared__ double sh[4];
+	double acc[MAX_DOTS];
+	for(int i = 0; i < MAX_DOTS; i++) acc[i] = 0.0;
+	const long stride = (long)gridDim.x*blockDim.x;
+	for(long e = (long)blockIdx.x*blockDim.x + threadIdx.x; e < N; e += stride) {
+		const double we = w[e];
+		for(int i = 0; i < MAX_DOTS; i++)
+			if(i < nv) acc[i] += V[i*ldv + e]*we;
+	}
+	for(int i = 0; i < MAX_DOTS; i++) {
+		if(i >= nv) break;
+		const double s = block_sum(acc[i], sh);
+		if(threadIdx.x == 0) partial[(size_t)blockIdx.x*MAX_DOTS + i] = s;
+	}
+}
+
+/// out[i] = sum over blocks of partial[b][i], i < nv (one block, fixed order).
+__global__ __launch_bounds__(BLOCK)
+void k_reduce(int nblk, const double* __restrict__ partial, int nv, double* __restrict__ out)
+{
+	__shared__ double sh[4];
+	for(int i = 0; i < nv; i++) {
+		double v = 0.0;
+		for(int b = threadIdx.x; b < nblk; b += blockDim.x) v += partial[(size_t)b*MAX_DOTS + i];
+		const double s = block_sum(v, sh);
+		if(threadIdx.x == 0) out[i] = s;
+		__syncthreads();
+	}
+}
+
+/// w -= sum_i h_i V_i (classical Gram-Schmidt update) and partial[b][0] = block part of |w|^2.
+__global__ __launch_bounds__(BLOCK)
+void k_gs_update(long N, const double* __restrict__ V, long ldv, int nv,
+                 const double* __restrict__ h, double* __restrict__ w,
+                 double* __restrict__ partial)
+{
+	__shared__ double sh[4];
+	__shared__ double hs[MAX_DOTS];
+	if(threadIdx.x < MAX_DOTS) hs[threadIdx.x] = threadIdx.x < nv ? h[threadIdx.x] : 0.0;
+	__syncthreads();
+	double acc = 0.0;
+	const long stride = (long)gridDim.x*blockDim.x;
+	for(long e = (long)blockIdx.x*blockDim.x + threadIdx.x; e < N; e += stride) {
+		double x = w[e];
+		for(int i = 0; i < MAX_DOTS; i++)
+			if(i < nv) x -= hs[i]*V[i*ldv + e];
+		w[e] = x;
+		acc += x*x;
+	}
+	const double s = block_sum(acc, sh);
+	if(threadIdx.x == 0) partial[(size_t)blockIdx.x*MAX_DOTS] = s;
+}
+
+/// w /= sqrt(*nrm2)
+__global__ __launch_bounds__(BLOCK)
+void k_normalise(long N, double* __restrict__ w, const double* __restrict__ nrm2)
+{
+	const double h = sqrt(nrm2[0]);
+	const long stride = (long)gridDim.x*blockDim.x;
+	for(long e = (long)blockIdx.x*blockDim.x + threadIdx.x; e < N; e += stride) w[e] /= h;
+}
+
+/// out = b / beta, beta = sqrt(*nrm2)
+__global__ __launch_bounds__(BLOCK)
+void k_scale_copy(long N, const double* __restrict__ b, const double* __restrict__ nrm2,
+                  double* __restrict__ out)
+{
+	const double h = sqrt(nrm2[0]);
+	const long stride = (long)gridDim.x*blockDim.x;
+	for(long e = (long)blockIdx.x*blockDim.x + threadIdx.x; e < N; e += stride)
+		out[e] = b[e]/h;
+}
+
+/// x = sum_k y_k Z_k, k < nv
+__global__ __launch_bounds__(BLOCK)
+void k_lincomb(long N, const double* __restrict__ Z, long ldz, int nv,
+               const double* __restrict__ y, double* __restrict__ x)
+{
+	__shared__ double ys[MAX_DOTS];
+	if(threadIdx.x < MAX_DOTS) ys[threadIdx.x] = threadIdx.x < nv ? y[threadIdx.x] : 0.0;
+	__syncthreads();
+	const long stride = (long)gridDim.x*blockDim.x;
+	for(long e = (long)blockIdx.x*blockDim.x + threadIdx.x; e < N; e += stride) {
+		double s = 0.0;
+		for(int k = 0; k < MAX_DOTS; k++)
+			if(k < nv) s += ys[k]*Z[k*ldz + e];
+		x[e] = s;
+	}
+}
+
+/// partial[b][0] = block part of sum_c r[c][3]^2 area[c] (the reference's energy-residual
+/// norm, src/ode/aodesolver.cpp:516-530).
+__global__ __launch_bounds__(BLOCK)
+void k_resnorm(int n, const double* __restrict__ r, const double* __restrict__ area,
+               double* __restrict__ partial)
+{
+	__shared__ double sh[4];
+	double acc = 0.0;
+	for(int c = blockIdx.x*blockDim.x + threadIdx.x; c < n; c += gridDim.x*blockDim.x) {
+		const double e = r[4*c + 3];
+		acc += e*e*area[c];
+	}
+	const double s = block_sum(acc, sh);
+	if(threadIdx.x == 0) partial[(size_t)blockIdx.x*MAX_DOTS] = s;
+}
+
+/* ------------------------------------------------------------ matrix-free / updates ---- */
+
+/// aux = u + (eps/|x|) x with |x|^2 in *xx
+__global__ __launch_bounds__(BLOCK)
+void k_mf_prep(long N, const double* __restrict__ u, const double* __restrict__ x,
+               const double* __restrict__ xx, double eps, double* __restrict__ aux)
+{
+	const double pert = eps/sqrt(xx[0]);
+	const long stride = (long)gridDim.x*blockDim.x;
+	for(long e = (long)blockIdx.x*blockDim.x + threadIdx.x; e < N; e += stride)
+		aux[e] = u[e] + pert*x[e];
+}
+
+/// y = dtm x + (r_u - r_aux)/pert, where r_u and r_aux hold -r() at u and at u + pert x.
+__global__ __launch_bounds__(BLOCK)
+void k_mf_finish(int n, const double* __restrict__ x, const double* __restrict__ dtm,
+                 const double* __restrict__ ru, const double* __restrict__ raux,
+                 const double* __restrict__ xx, double eps, double* __restrict__ y)
+{
+	const double pert = eps/sqrt(xx[0]);
+	for(int e = blockIdx.x*blockDim.x + threadIdx.x; e < 4*n; e += gridDim.x*blockDim.x)
+		y[e] = dtm[e >> 2]*x[e] + (-raux[e] + ru[e])/pert;
+}
+
+/// u += omega du with the local relaxation factor of FlowSimpleUpdate
+/// (src/ode/nonlinearrelaxation.cpp:25-38), or omega = 1 for the FULL update. The pressure
+/// change is IdealGasPhysics::getDeltaPressureFromConserved (aphysics_defs.hpp:65-81), whose
+/// sum runs over conserved components 2 and 3 (y-momentum and energy), not over the two
+/// momenta; that sum is kept as the reference has it.
+__global__ __launch_bounds__(BLOCK)
+void k_update_implicit(int n, double* __restrict__ u, const double* __restrict__ du,
+                       int full, double min_relax, double gamma)
+{
+	const int c = blockIdx.x*blockDim.x + threadIdx.x;
+	if(c >= n) return;
+	double uc[4], d[4];
+	load4(u + 4*c, uc);
+	load4(du + 4*c, d);
+	double om = 1.0;
+	if(!full) {
+		double un[4];
+		for(int i = 0; i < 4; i++) un[i] = uc[i] + d[i];
+		double dp = 0.0;
+		for(int i = 2; i < 4; i++)
+			dp -= (uc[i] + un[i])*(uc[0] + un[0])/2.0*d[i] - (un[i]*un[i] + uc[i]*uc[i])/2.0*d[0];
+		dp = (gamma - 1.0)*(d[3] - 1.0/(2*uc[0]*un[0])*dp);
+		const double p = (gamma - 1.0)*(uc[3] - 0.5*(uc[1]*uc[1] + uc[2]*uc[2])/uc[0]);
+		const double danger = fmax(fabs(dp)/p, fabs(d[0])/uc[0]);
+		om = danger < 1.0 - min_relax ? 1.0 - danger : min_relax;
+	}
+	for(int i = 0; i < 4; i++) uc[i] += om*d[i];
+	store4(u + 4*c, uc);
+}
+
+/// u += cfl dt/area r (SteadyForwardEulerSolver::solve, src/ode/aodesolver.cpp:200-212).
+__global__ __launch_bounds__(BLOCK)
+void k_update_explicit(int n, double* __restrict__ u, const double* __restrict__ r,
+                       const double* __restrict__ dt, const double* __restrict__ area,
+                       double cfl)
+{
+	const int c = blockIdx.x*blockDim.x + threadIdx.x;
+	if(c >= n) return;
+	const double f = cfl*dt[c]*1.0/area[c];
+	for(int i = 0; i < 4; i++) u[4*c + i] += f*r[4*c + i];
+}
+
+/* ------------------------------------------------------------------ launchers ---------- */
+
+static inline int quad_blocks(int n) { return (n + CELLS_PER_BLOCK - 1)/CELLS_PER_BLOCK; }
+static inline int thread_blocks(long n) { return (int)((n + BLOCK - 1)/BLOCK); }
+static inline int stream_blocks(long N)
+{
+	const long b = (N + BLOCK - 1)/BLOCK;
+	return (int)(b < 4096 ? (b > 0 ? b : 1) : 4096);
+}
+
+void launch_spmv(const fvx_matrix_s& A, const double* x, double* y, hipStream_t s)
+{
+	const int n = A.m->nelem;
+	if(A.prec == FVX_PREC_F32)
+		hipLaunchKernelGGL(k_spmv<float>, dim3(quad_blocks(n)), dim3(BLOCK), 0, s, n,
+		                   A.f32(), A.m->d_nbr.p, x, y);
+	else
+		hipLaunchKernelGGL(k_spmv<double>, dim3(quad_blocks(n)), dim3(BLOCK), 0, s, n,
+		                   A.f64(), A.m->d_nbr.p, x, y);
+	FVX_HIP(hipGetLastError());
+}
+
+void launch_pseudotime(fvx_matrix_s& A, const double* dt, double cfl, double* dtm,
+                       hipStream_t s)
+{
+	const int n = A.m->nelem;
+	if(A.prec == FVX_PREC_F32)
+		hipLaunchKernelGGL(k_pseudotime<float>, dim3(quad_blocks(n)), dim3(BLOCK), 0, s, n,
+		                   A.f32(), dt, A.m->d_area.p, cfl, dtm);
+	else
+		hipLaunchKernelGGL(k_pseudotime<double>, dim3(quad_blocks(n)), dim3(BLOCK), 0, s, n,
+		                   A.f64(), dt, A.m->d_area.p, cfl, dtm);
+	FVX_HIP(hipGetLastError());
+}
+
+template <class VTA, class VTF>
+static void pc_setup_t(fvx_pc_s& p, const VTA* vals, VTF* fac, hipStream_t s)
+{
+	const int n = p.m->nelem;
+	if(p.type == FVX_PC_JACOBI || p.type == FVX_PC_SGS) {
+		hipLaunchKernelGGL((k_invert_diag<VTA, VTF>), dim3(thread_blocks(n)), dim3(BLOCK), 0, s,
+		                   n, vals, fac, p.type == FVX_PC_SGS ? 1 : 0);
+		FVX_HIP(hipGetLastError());
+	} else if(p.type == FVX_PC_ILU0) {
+		for(int c = 0; c < p.ncol; c++) {
+			const int cnt = p.coff[c+1] - p.coff[c];
+			if(cnt == 0) continue;
+			hipLaunchKernelGGL((k_ilu_factor<VTA, VTF>), dim3(thread_blocks(cnt)), dim3(BLOCK), 0,
+			                   s, cnt, p.d_order.p + p.coff[c], vals, fac, p.m->d_nbr.p,
+			                   p.d_lowmask.p, p.d_colour.p, n);
+			FVX_HIP(hipGetLastError());
+		}
+	}
+}
+
+void launch_pc_setup(fvx_pc_s& p, const fvx_matrix_s& A, hipStream_t s)
+{
+	if(p.type == FVX_PC_NONE) return;
+	if(A.prec == FVX_PREC_F32) {
+		if(p.prec == FVX_PREC_F32) pc_setup_t(p, A.f32(), p.f32(), s);
+		else pc_setup_t(p, A.f32(), p.f64(), s);
+	} else {
+		if(p.prec == FVX_PREC_F32) pc_setup_t(p, A.f64(), p.f32(), s);
+		else pc_setup_t(p, A.f64(), p.f64(), s);
+	}
+}
+
+template <class VTF>
+static void pc_apply_t(const fvx_pc_s& p, const VTF* fac, const double* r, double* z,
+                       hipStream_t s)
+{
+	const int n = p.m->nelem;
+	const int* nbr = p.m->d_nbr.p;
+	if(p.type == FVX_PC_JACOBI) {
+		hipLaunchKernelGGL(k_jacobi_apply<VTF>, dim3(quad_blocks(n)), dim3(BLOCK), 0, s, n, fac,
+		                   r, z);
+		FVX_HIP(hipGetLastError());
+		return;
+	}
+	for(int c = 0; c < p.ncol; c++) {
+		const int cnt = p.coff[c+1] - p.coff[c];
+		if(cnt == 0) continue;
+		const int* list = p.d_order.p + p.coff[c];
+		if(p.type == FVX_PC_SGS)
+			hipLaunchKernelGGL((k_sgs_colour<VTF, false>), dim3(quad_blocks(cnt)), dim3(BLOCK), 0,
+			                   s, cnt, list, fac, nbr, p.d_lowmask.p, r, z);
+		else
+			hipLaunchKernelGGL(k_ilu_fwd<VTF>, dim3(quad_blocks(cnt)), dim3(BLOCK), 0, s, cnt,
+			                   list, fac, nbr, p.d_lowmask.p, r, z);
+		FVX_HIP(hipGetLastError());
+	}
+	for(int c = p.ncol - 1; c >= 0; c--) {
+		const int cnt = p.coff[c+1] - p.coff[c];
+		if(cnt == 0) continue;
+		const int* list = p.d_order.p + p.coff[c];
+		if(p.type == FVX_PC_SGS)
+			hipLaunchKernelGGL((k_sgs_colour<VTF, true>), dim3(quad_blocks(cnt)), dim3(BLOCK), 0,
+			                   s, cnt, list, fac, nbr, p.d_lowmask.p, r, z);
+		else
+			hipLaunchKernelGGL(k_ilu_bwd<VTF>, dim3(quad_blocks(cnt)), dim3(BLOCK), 0, s, cnt,
+			                   list, fac, nbr, p.d_lowmask.p, z);
+		FVX_HIP(hipGetLastError());
+	}
+}
+
+void launch_pc_apply(const fvx_pc_s& p, const double* r, double* z, hipStream_t s)
+{
+	if(p.type == FVX_PC_NONE) {
+		FVX_HIP(hipMemcpyAsync(z, r, sizeof(double)*4*(size_t)p.m->nelem,
+		                       hipMemcpyDeviceToDevice, s));
+		return;
+	}
+	if(p.prec == FVX_PREC_F32) pc_apply_t(p, p.f32(), r, z, s);
+	else pc_apply_t(p, p.f64(), r, z, s);
+}
+
+int reduction_blocks() { return RED_BLOCKS; }
+int max_dots() { return MAX_DOTS; }
+
+void launch_mdot(long N, const double* V, long ldv, int nv, const double* w, double* partial,
+                 double* out, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_mdot, dim3(RED_BLOCKS), dim3(BLOCK), 0, s, N, V, ldv, nv, w, partial);
+	hipLaunchKernelGGL(k_reduce, dim3(1), dim3(BLOCK), 0, s, RED_BLOCKS, partial, nv, out);
+	FVX_HIP(hipGetLastError());
+}
+
+void launch_gs_update(long N, const double* V, long ldv, int nv, const double* h, double* w,
+                      double* partial, double* nrm2, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_gs_update, dim3(RED_BLOCKS), dim3(BLOCK), 0, s, N, V, ldv, nv, h, w,
+	                   partial);
+	hipLaunchKernelGGL(k_reduce, dim3(1), dim3(BLOCK), 0, s, RED_BLOCKS, partial, 1, nrm2);
+	FVX_HIP(hipGetLastError());
+}
+
+void launch_normalise(long N, double* w, const double* nrm2, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_normalise, dim3(stream_blocks(N)), dim3(BLOCK), 0, s, N, w, nrm2);
+	FVX_HIP(hipGetLastError());
+}
+
+void launch_scale_copy(long N, const double* b, const double* nrm2, double* out, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_scale_copy, dim3(stream_blocks(N)), dim3(BLOCK), 0, s, N, b, nrm2, out);
+	FVX_HIP(hipGetLastError());
+}
+
+void launch_lincomb(long N, const double* Z, long ldz, int nv, const double* y, double* x,
+                    hipStream_t s)
+{
+	hipLaunchKernelGGL(k_lincomb, dim3(stream_blocks(N)), dim3(BLOCK), 0, s, N, Z, ldz, nv, y, x);
+	FVX_HIP(hipGetLastError());
+}
+
+void launch_resnorm(int n, const double* r, const double* area, double* partial, double* out,
+                    hipStream_t s)
+{
+	hipLaunchKernelGGL(k_resnorm, dim3(RED_BLOCKS), dim3(BLOCK), 0, s, n, r, area, partial);
+	hipLaunchKernelGGL(k_reduce, dim3(1), dim3(BLOCK), 0, s, RED_BLOCKS, partial, 1, out);
+	FVX_HIP(hipGetLastError());
+}
+
+void launch_mf_prep(long N, const double* u, const double* x, const double* xx, double eps,
+                    double* aux, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_mf_prep, dim3(stream_blocks(N)), dim3(BLOCK), 0, s, N, u, x, xx, eps,
+	                   aux);
+	FVX_HIP(hipGetLastError());
+}
+
+void launch_mf_finish(int n, const double* x, const double* dtm, const double* ru,
+                      const double* raux, const double* xx, double eps, double* y,
+                      hipStream_t s)
+{
+	hipLaunchKernelGGL(k_mf_finish, dim3(stream_blocks(4L*n)), dim3(BLOCK), 0, s, n, x, dtm, ru,
+	                   raux, xx, eps, y);
+	FVX_HIP(hipGetLastError());
+}
+
+void launch_update_implicit(int n, double* u, const double* du, int full, double min_relax,
+                            double gamma, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_update_implicit, dim3(thread_blocks(n)), dim3(BLOCK), 0, s, n, u, du,
+	                   full, min_relax, gamma);
+	FVX_HIP(hipGetLastError());
+}
+
+void launch_update_explicit(int n, double* u, const double* r, const double* dt,
+                            const double* area, double cfl, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_update_explicit, dim3(thread_blocks(n)), dim3(BLOCK), 0, s, n, u, r, dt,
+	                   area, cfl);
+	FVX_HIP(hipGetLastError());
+}
+
+}

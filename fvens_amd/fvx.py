@@ -89,9 +89,15 @@ def _ptr(a, ct):
     return a.ctypes.data_as(C.POINTER(ct))
 
 
-def _dptr(t):
-    """Device pointer of a torch tensor (must be contiguous)."""
-    assert t.is_contiguous()
+def _dptr(t, numel=None):
+    """Device pointer of a contiguous float64 device tensor, checked against the element
+    count the kernels will touch before anything is launched."""
+    if not (t.is_cuda and t.is_contiguous()):
+        raise ValueError("expected a contiguous device tensor")
+    if numel is not None:
+        import torch
+        if t.dtype != torch.float64 or t.numel() != numel:
+            raise ValueError(f"expected {numel} float64 elements, got {t.numel()} {t.dtype}")
     return C.c_void_p(t.data_ptr())
 
 
@@ -198,15 +204,22 @@ class FlowFV:
 
     def compute_residual(self, u, r, gettimesteps=False, dt=None):
         """Adds -r(u) into r; fills dt with local time steps if requested (device tensors)."""
-        check(lib().fvx_compute_residual(self.h, _dptr(u), _dptr(r), int(gettimesteps),
-                                         _dptr(dt) if dt is not None else C.c_void_p(0),
+        n = self.mesh.nelem
+        if gettimesteps and dt is None:
+            raise ValueError("time steps requested without an output tensor")
+        check(lib().fvx_compute_residual(self.h, _dptr(u, 4 * n), _dptr(r, 4 * n),
+                                         int(gettimesteps),
+                                         _dptr(dt, n) if dt is not None else C.c_void_p(0),
                                          _stream()))
 
     def get_gradients(self, u, grads):
-        check(lib().fvx_get_gradients(self.h, _dptr(u), _dptr(grads), _stream()))
+        n = self.mesh.nelem
+        check(lib().fvx_get_gradients(self.h, _dptr(u, 4 * n), _dptr(grads, 8 * n), _stream()))
 
     def get_primitive_gradients(self, u, grads):
-        check(lib().fvx_get_primitive_gradients(self.h, _dptr(u), _dptr(grads), _stream()))
+        n = self.mesh.nelem
+        check(lib().fvx_get_primitive_gradients(self.h, _dptr(u, 4 * n), _dptr(grads, 8 * n),
+                                                _stream()))
 
     KERNELS = {"gradients": 0, "flux": 1, "jacobian": 2, "pc": 3}
 
@@ -222,6 +235,143 @@ class FlowFV:
 
     def functionals(self, u, wall_tag=2):
         out = np.zeros(4)
-        check(lib().fvx_functionals(self.h, _dptr(u), int(wall_tag), _ptr(out, C.c_double),
+        check(lib().fvx_functionals(self.h, _dptr(u, 4 * self.mesh.nelem), int(wall_tag),
+                                    _ptr(out, C.c_double),
                                     _stream()))
         return dict(CL=out[0], CDp=out[1], CDsf=out[2], entropy=out[3])
+
+    # ---- implicit solve -------------------------------------------------------------------
+
+    def assemble_jacobian(self, u, A: "Matrix"):
+        """Overwrites A with the first-order approximate Jacobian at u."""
+        check(lib().fvx_assemble_jacobian(self.h, _dptr(u, 4 * self.mesh.nelem), A.h, _stream()))
+
+    def add_pseudotime(self, A: "Matrix", dt, cfl, dtm):
+        n = self.mesh.nelem
+        check(lib().fvx_add_pseudotime(self.h, A.h, _dptr(dt, n), C.c_double(cfl), _dptr(dtm, n),
+                                       _stream()))
+
+    def matfree_apply(self, u, r_u, dtm, x, y, eps=1e-7):
+        n = self.mesh.nelem
+        check(lib().fvx_matfree_apply(self.h, _dptr(u, 4 * n), _dptr(r_u, 4 * n), _dptr(dtm, n),
+                                      _dptr(x, 4 * n), _dptr(y, 4 * n), C.c_double(eps),
+                                      _stream()))
+
+    def steady_solve(self, u, raise_on_failure=True, **kw):
+        """Runs the pseudo-time loop on the device tensor u (updated in place); returns the
+        statistics as a dict. Keyword arguments override solver_cfg() defaults."""
+        cfg = solver_cfg(**kw)
+        st = SolveStats()
+        rc = lib().fvx_steady_solve(self.h, C.byref(cfg), _dptr(u, 4 * self.mesh.nelem),
+                                    C.byref(st), _stream())
+        out = {f: getattr(st, f) for f, _ in SolveStats._fields_}
+        out["rc"] = rc
+        if raise_on_failure:
+            check(rc)
+        return out
+
+
+PREC = {"f64": 0, "f32": 1}
+PC = {"none": 0, "jacobi": 1, "sgs": 2, "ilu0": 3}
+
+
+class SolverCfg(C.Structure):
+    _fields_ = [("cflinit", C.c_double), ("cflfin", C.c_double), ("tol", C.c_double),
+                ("maxiter", C.c_int), ("implicit", C.c_int), ("fullupdate", C.c_int),
+                ("min_relax", C.c_double), ("pc", C.c_int), ("pc_precision", C.c_int),
+                ("jac_precision", C.c_int), ("krylov_restart", C.c_int),
+                ("krylov_rtol", C.c_double), ("matrix_free", C.c_int), ("mf_eps", C.c_double),
+                ("fixed_norm", C.c_double)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("steps", C.c_int), ("lin_iters", C.c_int), ("converged", C.c_int),
+                ("status", C.c_int), ("final_rel", C.c_double), ("final_abs", C.c_double),
+                ("first_norm", C.c_double), ("walltime", C.c_double)]
+
+
+def solver_cfg(cflinit=500.0, cflfin=5000.0, tol=1e-7, maxiter=170, implicit=True,
+               fullupdate=False, min_relax=0.1, pc="ilu0", pc_precision="f64",
+               jac_precision="f64", krylov_restart=30, krylov_rtol=1e-1, matrix_free=False,
+               mf_eps=1e-7, fixed_norm=0.0):
+    """Defaults: testcases/naca0012/transonic-sanity-test-muscl.ctrl main solve with the
+    linear-solver options of testcases/defaults.solverc."""
+    c = SolverCfg()
+    c.cflinit, c.cflfin, c.tol, c.maxiter = cflinit, cflfin, tol, maxiter
+    c.implicit, c.fullupdate, c.min_relax = int(implicit), int(fullupdate), min_relax
+    c.pc, c.pc_precision, c.jac_precision = PC[pc], PREC[pc_precision], PREC[jac_precision]
+    c.krylov_restart, c.krylov_rtol = krylov_restart, krylov_rtol
+    c.matrix_free, c.mf_eps, c.fixed_norm = int(matrix_free), mf_eps, fixed_norm
+    return c
+
+
+class Matrix:
+    """Device block matrix (SeqBAIJ, bs = 4) on a mesh's cell graph."""
+
+    def __init__(self, mesh: Mesh, precision="f64"):
+        self.mesh = mesh
+        h = C.c_void_p()
+        check(lib().fvx_matrix_create(mesh.h, PREC[precision], C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.fvx_matrix_destroy(self.h)
+            self.h = None
+
+    def apply(self, x, y):
+        n = self.mesh.nelem
+        check(lib().fvx_matrix_apply(self.h, _dptr(x, 4 * n), _dptr(y, 4 * n), _stream()))
+
+    def to_bsr(self):
+        """(rowptr, colind, vals[nnzb,4,4]) on the host, columns sorted."""
+        import torch
+        torch.cuda.synchronize()
+        nnzb = C.c_int()
+        check(lib().fvx_matrix_bsr_nnzb(self.h, C.byref(nnzb)))
+        rowptr = np.zeros(self.mesh.nelem + 1, np.int32)
+        colind = np.zeros(nnzb.value, np.int32)
+        vals = np.zeros((nnzb.value, 4, 4))
+        check(lib().fvx_matrix_export_bsr(self.h, _ptr(rowptr, C.c_int), _ptr(colind, C.c_int),
+                                          _ptr(vals, C.c_double)))
+        return rowptr, colind, vals
+
+
+class Preconditioner:
+    """Device preconditioner (none / block Jacobi / multicolour SGS / multicolour BILU(0))."""
+
+    def __init__(self, mesh: Mesh, kind="ilu0", precision="f64"):
+        self.mesh = mesh
+        h = C.c_void_p()
+        check(lib().fvx_pc_create(mesh.h, PC[kind], PREC[precision], C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.fvx_pc_destroy(self.h)
+            self.h = None
+
+    def setup(self, A: Matrix):
+        check(lib().fvx_pc_setup(self.h, A.h, _stream()))
+
+    def apply(self, r, z):
+        n = self.mesh.nelem
+        check(lib().fvx_pc_apply(self.h, _dptr(r, 4 * n), _dptr(z, 4 * n), _stream()))
+
+    def ordering(self):
+        col = np.zeros(self.mesh.nelem, np.int32)
+        nc = C.c_int()
+        check(lib().fvx_pc_ordering(self.h, _ptr(col, C.c_int), C.byref(nc)))
+        return col, nc.value
+
+
+def linear_solve(A: Matrix, pc: Preconditioner, b, x, restart=30, rtol=1e-1):
+    """FGMRES(restart) from zero; returns the iteration count."""
+    it = C.c_int()
+    n = A.mesh.nelem
+    if pc.mesh is not A.mesh:
+        raise ValueError("preconditioner and matrix belong to different meshes")
+    check(lib().fvx_linear_solve(A.h, pc.h, _dptr(b, 4 * n), _dptr(x, 4 * n), int(restart),
+                                 C.c_double(rtol),
+                                 C.byref(it), _stream()))
+    return it.value

@@ -129,6 +129,92 @@ fvx_status fvx_get_primitive_gradients(fvx_disc d, const double* u, double* grad
 fvx_status fvx_functionals(fvx_disc d, const double* u, int wall_tag, double* out,
                            void* stream);
 
+/* ---- implicit solve: system matrix, preconditioners, Krylov solver, pseudo-time loop ---- */
+
+/* Storage precision of matrices and preconditioner factors; vectors are always double. */
+enum { FVX_PREC_F64 = 0, FVX_PREC_F32 = 1 };
+/* Preconditioners (PETSc -pc_type none / pbjacobi / sor / bjacobi+ilu on one rank). SGS and
+ * ILU0 run in a multicolour ordering of the cells (greedy colouring in cell order; cells of
+ * one colour share no face), which makes each sweep of a colour fully parallel. */
+enum { FVX_PC_NONE = 0, FVX_PC_JACOBI = 1, FVX_PC_SGS = 2, FVX_PC_ILU0 = 3 };
+
+typedef struct fvx_matrix_s* fvx_matrix;
+typedef struct fvx_pc_s* fvx_pc;
+
+/* Block-sparse 4x4 matrix on the cell graph of a mesh: the SeqBAIJ (bs = 4) system matrix of
+ * setupSystemMatrix (src/linalg/alinalg.cpp:90-119). Internally one diagonal block and one
+ * block per cell face. */
+fvx_status fvx_matrix_create(fvx_mesh m, int precision, fvx_matrix* out);
+fvx_status fvx_matrix_destroy(fvx_matrix A);
+/* Spatial::assemble_jacobian (src/spatial/aspatial.cpp:242-340) preceded by the caller's
+ * MatZeroEntries (src/ode/aodesolver.cpp:456): overwrites A with the first-order approximate
+ * Jacobian dr/du at u (device pointer). */
+fvx_status fvx_assemble_jacobian(fvx_disc d, const double* u, fvx_matrix A, void* stream);
+/* SteadyBackwardEulerSolver::addPseudoTimeTerm (src/ode/aodesolver.cpp:300-329):
+ * dtm[i] = area_i/(cfl dt[i]) and A_ii += dtm[i] I. dt, dtm: device [ncell]. */
+fvx_status fvx_add_pseudotime(fvx_disc d, fvx_matrix A, const double* dt, double cfl,
+                              double* dtm, void* stream);
+/* MatMult: y = A x (device vectors [ncell][4]). */
+fvx_status fvx_matrix_apply(fvx_matrix A, const double* x, double* y, void* stream);
+/* Number of nonzero blocks of the equivalent BAIJ matrix (diagonal + interior faces). */
+fvx_status fvx_matrix_bsr_nnzb(fvx_matrix A, int* nnzb);
+/* Host copy as block CSR with sorted columns in cell order: rowptr[ncell+1],
+ * colind[nnzb], vals[nnzb][4][4] row-major. Synchronises the device. */
+fvx_status fvx_matrix_export_bsr(fvx_matrix A, int* rowptr, int* colind, double* vals);
+
+/* PCCreate/PCSetType for a matrix of this mesh; precision = storage of the factors. */
+fvx_status fvx_pc_create(fvx_mesh m, int type, int precision, fvx_pc* out);
+fvx_status fvx_pc_destroy(fvx_pc p);
+/* PCSetUp: factorises / inverts from the current values of A. */
+fvx_status fvx_pc_setup(fvx_pc p, fvx_matrix A, void* stream);
+/* PCApply: z = M^-1 r (device vectors [ncell][4]). */
+fvx_status fvx_pc_apply(fvx_pc p, const double* r, double* z, void* stream);
+/* Host copy of the colour of every cell and the number of colours of the PC's ordering. */
+fvx_status fvx_pc_ordering(fvx_pc p, int* colour, int* ncolours);
+
+/* KSPSolve with -ksp_type fgmres (src/ode/aodesolver.cpp:483; options
+ * testcases/defaults.solverc): one restart cycle of at most `restart` iterations of right-
+ * preconditioned flexible GMRES with classical Gram-Schmidt from x = 0, stopping when
+ * ||b - A x|| <= rtol ||b||. x is overwritten; iters receives the iteration count. */
+fvx_status fvx_linear_solve(fvx_matrix A, fvx_pc p, const double* b, double* x, int restart,
+                            double rtol, int* iters, void* stream);
+
+/* MatrixFreeSpatialJacobian::apply (src/linalg/alinalg.cpp:142-233):
+ * y = dtm x + (r(u + eps x/||x||) - r(u)) ||x||/eps, where r_u holds the caller's residual
+ * buffer at u in the engine's sign convention (-r(u), as fvx_compute_residual adds it). */
+fvx_status fvx_matfree_apply(fvx_disc d, const double* u, const double* r_u, const double* dtm,
+                             const double* x, double* y, double eps, void* stream);
+
+/* SteadySolverConfig (src/ode/aodesolver.hpp:16-42) plus the linear-solver options. */
+typedef struct {
+	double cflinit, cflfin;   /* CFL ramp bounds */
+	double tol;               /* stop when ||r||/||r_0|| <= tol */
+	int maxiter;              /* pseudo-time step limit */
+	int implicit;             /* 1 backward Euler, 0 forward Euler */
+	int fullupdate;           /* 1 FULL update, 0 ROBUST_FLOW local relaxation */
+	double min_relax;         /* minimum local relaxation factor */
+	int pc;                   /* FVX_PC_* */
+	int pc_precision;         /* FVX_PREC_* of the preconditioner factors */
+	int jac_precision;        /* FVX_PREC_* of the assembled Jacobian */
+	int krylov_restart;       /* -ksp_max_it (one FGMRES cycle) */
+	double krylov_rtol;       /* -ksp_rtol */
+	int matrix_free;          /* 1: matrix-free operator, assembled Jacobian only in the PC */
+	double mf_eps;            /* matrix-free difference step */
+	double fixed_norm;        /* > 0: normalise residuals by this instead of the first norm */
+} fvx_solver_cfg;
+
+typedef struct {
+	int steps, lin_iters, converged, status;
+	double final_rel, final_abs, first_norm, walltime;
+} fvx_solve_stats;
+
+/* SteadyForwardEulerSolver::solve / SteadyBackwardEulerSolver::solve
+ * (src/ode/aodesolver.cpp:135-282, 363-638) run on the device: u (device [ncell][4]) is
+ * updated in place. Returns FVX_ERR_NONFINITE on a non-finite residual norm and
+ * FVX_ERR_NOT_CONVERGED when maxiter is reached; stats are filled in either case. */
+fvx_status fvx_steady_solve(fvx_disc d, const fvx_solver_cfg* cfg, double* u,
+                            fvx_solve_stats* stats, void* stream);
+
 /* ---- timing: the TimingData role of the reference (src/ode/aodesolver.hpp:46-67) ---- */
 
 /* Kernel ids for fvx_profile_read. */

@@ -1,0 +1,364 @@
+"""GPU parity of the implicit-solver path through the C ABI against the CPU oracle:
+Jacobian assembly, pseudo-time term, block SpMV, preconditioners (in the device's multicolour
+ordering, checked by running the oracle on the same permuted matrix), FGMRES, the matrix-free
+operator and whole pseudo-time solves.
+
+Tolerances (written per test): Jacobian blocks 1e-12 of the row scale; preconditioner outputs
+1e-11 (FP64 factors) / 2e-5 (FP32 factors); FGMRES at rtol 1e-11 vs a dense solve 1e-8;
+converged functionals CL/CD 1e-8 against the oracle's converged solution.
+"""
+import numpy as np
+import pytest
+
+import pyoracle as po
+from conftest import golden
+from fvens_amd import fvx, gmsh, meshgen, states
+
+pytestmark = pytest.mark.gpu
+
+BUMP_BCS = [(2, "slipwall"), (3, "inflowoutflow"), (4, "inflowoutflow")]
+CYL_BCS = [(2, "slipwall"), (4, "farfield")]
+VISC_BCS = [(2, "adiabaticwall", (0.1, 0.0)), (3, "extrapolation"), (4, "farfield")]
+ISO_BCS = [(2, "isothermalwall", (0.0, 1.05)), (3, "slipwall"), (4, "farfield")]
+VISC = dict(Tinf=288.15, Reinf=5000.0, Pr=0.72, viscous=True)
+
+
+def _setup(raw, phys, num, seed=20261202, L=1.0):
+    import torch
+    ocfg = po.make_config(**phys, **num)
+    pk = dict(phys)
+    pk["const_visc"] = pk.pop("constvisc", False)
+    od = po.Disc(po.Mesh(raw), ocfg)
+    gm = fvx.Mesh(raw)
+    gd = fvx.FlowFV(gm, fvx.physics_cfg(**pk), fvx.numerics_cfg(**num))
+    u = states.perturbed_freestream(raw, Minf=phys["Minf"], aoa_deg=phys.get("aoa_deg", 0.0),
+                                    L=L, seed=seed)
+    return od, gm, gd, u, torch.tensor(u, device="cuda")
+
+
+def _dense(rowptr, colind, vals, n):
+    A = np.zeros((4 * n, 4 * n))
+    for i in range(n):
+        for k in range(rowptr[i], rowptr[i + 1]):
+            A[4 * i:4 * i + 4, 4 * colind[k]:4 * colind[k] + 4] = vals[k]
+    return A
+
+
+def _row_rel(rowptr, vg, vo):
+    """max over blocks of |Jg - Jo| / max |J| over the block row"""
+    worst = 0.0
+    for i in range(len(rowptr) - 1):
+        s = slice(rowptr[i], rowptr[i + 1])
+        scale = max(np.abs(vo[s]).max(), 1e-300)
+        worst = max(worst, np.abs(vg[s] - vo[s]).max() / scale)
+    return worst
+
+
+JAC_CASES = [
+    ("bump", dict(Minf=0.5), BUMP_BCS, "ROE"),
+    ("bump", dict(Minf=0.5), BUMP_BCS, "HLLC"),
+    ("bump", dict(Minf=0.5), BUMP_BCS, "HLL"),
+    ("bump", dict(Minf=0.5), BUMP_BCS, "LLF"),
+    ("bump", dict(Minf=0.5), BUMP_BCS, "AUSM"),
+    ("2dcylinderhybrid.msh", dict(Minf=0.4, aoa_deg=3.0), CYL_BCS, "HLLC"),
+    ("testperiodic.msh", dict(Minf=0.5, aoa_deg=10.0, **VISC), VISC_BCS, "ROE"),
+    ("testperiodic.msh", dict(Minf=0.5, aoa_deg=10.0, constvisc=True, **VISC), VISC_BCS, "HLL"),
+    ("2dcylinderhybrid.msh", dict(Minf=0.3, aoa_deg=3.0, **VISC), ISO_BCS, "HLLC"),
+]
+
+
+def _raw(name):
+    if name == "bump":
+        return meshgen.mesh_k1()
+    return gmsh.read_gmsh2(golden(name))
+
+
+@pytest.mark.parametrize("meshname,phys,bcs,flux", JAC_CASES)
+def test_jacobian_parity(torch_cuda, meshname, phys, bcs, flux):
+    raw = _raw(meshname)
+    od, gm, gd, u, ut = _setup(raw, dict(phys, bcs=bcs),
+                               dict(flux=flux, gradient="LEASTSQUARES", recon="VANALBADA"))
+    A = fvx.Matrix(gm, "f64")
+    gd.assemble_jacobian(ut, A)
+    rp, ci, vg = A.to_bsr()
+    rpo, cio, vo = od.jacobian(u)
+    assert np.array_equal(rp, rpo) and np.array_equal(ci, cio)
+    rel = _row_rel(rp, vg, vo)
+    assert rel <= 1e-12, rel
+    # single precision storage is the rounding of the double-precision values
+    A32 = fvx.Matrix(gm, "f32")
+    gd.assemble_jacobian(ut, A32)
+    _, _, v32 = A32.to_bsr()
+    assert np.array_equal(v32, vg.astype(np.float32).astype(np.float64)) or \
+        np.abs(v32 - vg).max() <= 1e-7 * np.abs(vg).max()
+
+
+def test_jacobian_is_deterministic_and_pseudotime_term(torch_cuda):
+    import torch
+    raw = meshgen.mesh_k1()
+    od, gm, gd, u, ut = _setup(raw, dict(Minf=0.5, bcs=BUMP_BCS), dict(flux="ROE"))
+    A = fvx.Matrix(gm)
+    gd.assemble_jacobian(ut, A)
+    rp, ci, v1 = A.to_bsr()
+    gd.assemble_jacobian(ut, A)
+    _, _, v2 = A.to_bsr()
+    assert np.array_equal(v1, v2)
+    n = raw.nelem
+    r = torch.zeros((n, 4), dtype=torch.float64, device="cuda")
+    dt = torch.zeros(n, dtype=torch.float64, device="cuda")
+    dtm = torch.zeros(n, dtype=torch.float64, device="cuda")
+    gd.compute_residual(ut, r, True, dt)
+    cfl = 37.5
+    gd.add_pseudotime(A, dt, cfl, dtm)
+    _, _, v3 = A.to_bsr()
+    _, _, area, _ = gm.faces()
+    dtm_ref = area / (cfl * dt.cpu().numpy())
+    assert np.allclose(dtm.cpu().numpy(), dtm_ref, rtol=1e-15, atol=0)
+    for i in range(n):
+        k = rp[i] + list(ci[rp[i]:rp[i + 1]]).index(i)
+        assert np.allclose(v3[k], v1[k] + dtm_ref[i] * np.eye(4), rtol=1e-14, atol=1e-14)
+
+
+def test_spmv_matches_host_bsr(torch_cuda):
+    import torch
+    raw = gmsh.read_gmsh2(golden("2dcylinderhybrid.msh"))
+    od, gm, gd, u, ut = _setup(raw, dict(Minf=0.4, aoa_deg=3.0, bcs=CYL_BCS), dict(flux="HLLC"))
+    for prec in ("f64", "f32"):
+        A = fvx.Matrix(gm, prec)
+        gd.assemble_jacobian(ut, A)
+        rp, ci, v = A.to_bsr()
+        x = np.random.default_rng(1).standard_normal((raw.nelem, 4))
+        y = torch.zeros((raw.nelem, 4), dtype=torch.float64, device="cuda")
+        A.apply(torch.tensor(x, device="cuda"), y)
+        ref = (_dense(rp, ci, v, raw.nelem) @ x.ravel()).reshape(-1, 4)
+        assert np.allclose(y.cpu().numpy(), ref, rtol=1e-13, atol=1e-13 * np.abs(ref).max())
+
+
+def _perm_bsr(rp, ci, v, perm):
+    """Matrix of rows/columns perm[new] = old, columns sorted."""
+    n = len(perm)
+    inv = np.empty(n, np.int64); inv[perm] = np.arange(n)
+    rows, cols, blks = [], [], []
+    for newi, oldi in enumerate(perm):
+        ent = sorted((inv[ci[k]], k) for k in range(rp[oldi], rp[oldi + 1]))
+        rows.append(len(ent))
+        cols += [c for c, _ in ent]
+        blks += [v[k] for _, k in ent]
+    return (np.concatenate([[0], np.cumsum(rows)]).astype(np.int32), np.array(cols, np.int32),
+            np.array(blks))
+
+
+@pytest.mark.parametrize("meshname", ["bump", "2dcylinderhybrid.msh"])
+@pytest.mark.parametrize("kind", ["jacobi", "sgs", "ilu0"])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_preconditioner_parity(torch_cuda, meshname, kind, prec):
+    import torch
+    raw = _raw(meshname)
+    bcs = BUMP_BCS if meshname == "bump" else CYL_BCS
+    od, gm, gd, u, ut = _setup(raw, dict(Minf=0.5, bcs=bcs), dict(flux="HLLC"))
+    n = raw.nelem
+    A = fvx.Matrix(gm, "f64")
+    gd.assemble_jacobian(ut, A)
+    r = torch.zeros((n, 4), dtype=torch.float64, device="cuda")
+    dt = torch.zeros(n, dtype=torch.float64, device="cuda")
+    dtm = torch.zeros(n, dtype=torch.float64, device="cuda")
+    gd.compute_residual(ut, r, True, dt)
+    gd.add_pseudotime(A, dt, 50.0, dtm)
+    P = fvx.Preconditioner(gm, kind, prec)
+    P.setup(A)
+    rhs = np.random.default_rng(3).standard_normal((n, 4))
+    z = torch.zeros((n, 4), dtype=torch.float64, device="cuda")
+    P.apply(torch.tensor(rhs, device="cuda"), z)
+    zg = z.cpu().numpy()
+    colour, ncol = P.ordering()
+    rp, ci, v = A.to_bsr()
+    if kind != "jacobi":
+        for i in range(n):   # a valid colouring: no face neighbours share a colour
+            nb = [c for c in ci[rp[i]:rp[i + 1]] if c != i]
+            assert all(colour[c] != colour[i] for c in nb)
+        assert ncol <= 5
+    perm = np.lexsort((np.arange(n), colour))
+    prp, pci, pv = _perm_bsr(rp, ci, v, perm)
+    zp = po.pc_apply_bsr(prp, pci, pv, kind, rhs[perm].ravel()).reshape(-1, 4)
+    zo = np.empty_like(zp); zo[perm] = zp
+    tol = 1e-11 if prec == "f64" else 2e-5
+    err = np.abs(zg - zo).max() / np.abs(zo).max()
+    assert err <= tol, (err, ncol)
+
+
+def test_linear_solve_matches_dense_solve(torch_cuda):
+    import torch
+    raw = gmsh.read_gmsh2(golden("2dcylinderhybrid.msh"))
+    od, gm, gd, u, ut = _setup(raw, dict(Minf=0.4, aoa_deg=3.0, bcs=CYL_BCS), dict(flux="ROE"))
+    n = raw.nelem
+    A = fvx.Matrix(gm)
+    gd.assemble_jacobian(ut, A)
+    r = torch.zeros((n, 4), dtype=torch.float64, device="cuda")
+    dt = torch.zeros(n, dtype=torch.float64, device="cuda")
+    dtm = torch.zeros(n, dtype=torch.float64, device="cuda")
+    gd.compute_residual(ut, r, True, dt)
+    gd.add_pseudotime(A, dt, 2.0, dtm)
+    rp, ci, v = A.to_bsr()
+    D = _dense(rp, ci, v, n)
+    b = np.random.default_rng(8).standard_normal(4 * n)
+    xref = np.linalg.solve(D, b)
+    for kind in ("ilu0", "sgs", "jacobi"):
+        P = fvx.Preconditioner(gm, kind)
+        P.setup(A)
+        x = torch.zeros((n, 4), dtype=torch.float64, device="cuda")
+        its = fvx.linear_solve(A, P, torch.tensor(b.reshape(-1, 4), device="cuda"), x,
+                               restart=31, rtol=1e-11)
+        xg = x.cpu().numpy().ravel()
+        assert its < 31, kind
+        assert np.linalg.norm(D @ xg - b) <= 1e-10 * np.linalg.norm(b), (kind, its)
+        assert np.abs(xg - xref).max() <= 1e-8 * np.abs(xref).max(), kind
+
+
+def test_matrix_free_operator(torch_cuda):
+    import torch
+    raw = meshgen.mesh_k1()
+    phys = dict(Minf=0.5, bcs=BUMP_BCS)
+    num = dict(flux="ROE", gradient="GREENGAUSS")
+    od, gm, gd, u, ut = _setup(raw, phys, num, L=3.0)
+    n = raw.nelem
+    r = torch.zeros((n, 4), dtype=torch.float64, device="cuda")
+    dt = torch.zeros(n, dtype=torch.float64, device="cuda")
+    gd.compute_residual(ut, r, True, dt)
+    dtm = torch.rand(n, dtype=torch.float64, device="cuda") + 1.0
+    x = np.random.default_rng(9).standard_normal((n, 4))
+    y = torch.zeros((n, 4), dtype=torch.float64, device="cuda")
+    eps = 1e-7
+    gd.matfree_apply(ut, r, dtm, torch.tensor(x, device="cuda"), y, eps)
+    # reference: the same difference quotient from oracle residuals
+    pert = eps / np.linalg.norm(x)
+    ro, _ = od.residual(u)
+    ra, _ = od.residual(u + pert * x)
+    yref = dtm.cpu().numpy()[:, None] * x + (ro - ra) / pert
+    err = np.abs(y.cpu().numpy() - yref).max() / np.abs(yref).max()
+    assert err <= 1e-6, err
+
+
+def test_explicit_steps_match_oracle(torch_cuda):
+    raw = meshgen.mesh_k1()
+    phys = dict(Minf=0.5, bcs=BUMP_BCS)
+    num = dict(flux="ROE", gradient="GREENGAUSS")
+    od, gm, gd, u, ut = _setup(raw, phys, num, L=3.0)
+    kw = dict(cflinit=0.5, cflfin=0.5, tol=1e-30, maxiter=5, implicit=False, fullupdate=True)
+    st = gd.steady_solve(ut, raise_on_failure=False, **kw)
+    uo, so = od.steady_solve(u, **{**kw, "implicit": 0, "fullupdate": 1})
+    assert st["steps"] == so["steps"] == 5
+    assert np.abs(ut.cpu().numpy() - uo).max() <= 1e-12 * np.abs(uo).max()
+    assert abs(st["final_abs"] / so["abs"] - 1) <= 1e-11
+
+
+def test_implicit_jacobi_steps_match_oracle(torch_cuda):
+    # same preconditioner and a fixed number of Krylov iterations: the updates agree to rounding
+    raw = meshgen.mesh_k1()
+    phys = dict(Minf=0.5, bcs=BUMP_BCS)
+    num = dict(flux="HLLC", gradient="LEASTSQUARES", recon="VENKATAKRISHNAN")
+    od, gm, gd, u, ut = _setup(raw, phys, num, L=3.0)
+    kw = dict(cflinit=20.0, cflfin=200.0, tol=1e-30, maxiter=4, fullupdate=False, min_relax=0.2,
+              pc="jacobi", krylov_restart=6, krylov_rtol=1e-14)
+    st = gd.steady_solve(ut, raise_on_failure=False, **kw)
+    uo, so = od.steady_solve(u, **{**kw, "fullupdate": 0})
+    assert st["steps"] == so["steps"] == 4 and st["lin_iters"] == so["lin_iters"] == 24
+    assert np.abs(ut.cpu().numpy() - uo).max() <= 1e-9 * np.abs(uo).max()
+
+
+def test_nonfinite_state_is_reported(torch_cuda):
+    # tests/flowpseudotime.cpp:55-64: a non-physical state must raise Numerical_error
+    raw = meshgen.mesh_k1()
+    od, gm, gd, u, ut = _setup(raw, dict(Minf=0.5, bcs=BUMP_BCS), dict(flux="ROE"))
+    ut[7, 3] = -1.0     # negative energy: pressure < 0, sound speed NaN
+    with pytest.raises(fvx.NumericalError):
+        gd.steady_solve(ut, maxiter=3)
+
+
+def _naca(precision="f64", pc="sgs", tol=1e-7, maxiter=170, matrix_free=False):
+    import torch
+    raw = gmsh.read_gmsh2(golden("naca0012luo.msh"))
+    gm = fvx.Mesh(raw)
+    phys = fvx.physics_cfg(Minf=0.8, aoa_deg=1.25, bcs=[(2, "slipwall"), (4, "farfield")])
+    d1 = fvx.FlowFV(gm, phys, fvx.numerics_cfg(flux="HLLC", gradient="NONE", order2=False))
+    d2 = fvx.FlowFV(gm, phys, fvx.numerics_cfg(flux="HLLC", gradient="LEASTSQUARES",
+                                               recon="VANALBADA", limiter_param=20.0))
+    u = torch.tensor(np.tile(d1.freestream(), (raw.nelem, 1)), device="cuda")
+    s1 = d1.steady_solve(u, raise_on_failure=False, cflinit=50.0, cflfin=1000.0, tol=1e-1,
+                         maxiter=20, min_relax=0.2, pc=pc, pc_precision=precision,
+                         jac_precision=precision)
+    s2 = d2.steady_solve(u, cflinit=500.0, cflfin=5000.0, tol=tol, maxiter=maxiter,
+                         min_relax=0.2, pc=pc, pc_precision=precision, jac_precision=precision,
+                         matrix_free=matrix_free)
+    return u, s1, s2, d2.functionals(u, 2)
+
+
+def test_naca0012_pipeline_against_reference_regression(torch_cuda):
+    u, s1, s2, f = _naca(tol=1e-7, maxiter=400)
+    with open(golden("naca0012_regr_MUSCL_LeastSquares_HLLC.txt")) as fh:
+        CL, CDp = (float(t) for t in fh.read().split()[:2])
+    # the regression values are themselves converged to 1e-7 only; the converged solution differs
+    # from them by ~5e-8 in CDp, so CDp is held to 1e-7 here and to 1e-8 against the oracle below
+    assert abs(f["CL"] - CL) / abs(CL) <= 1e-6
+    assert abs(f["CDp"] - CDp) / abs(CDp) <= 1e-7
+
+
+def test_naca0012_converged_functionals_match_oracle(torch_cuda):
+    # converged to 1e-11 the discrete solution no longer depends on the preconditioner or its
+    # ordering: the device's multicolour SGS (double and single precision Jacobian and factors)
+    # against the oracle's natural-order ILU(0)
+    import test_oracle_solver as tos
+    _, _, so, fo = tos.naca_regression_solve(pc="ilu0", tol=1e-11, maxiter=600)
+    assert so["converged"]
+    for prec in ("f64", "f32"):
+        _, _, s2, f = _naca(precision=prec, pc="sgs", tol=1e-11, maxiter=600)
+        assert s2["converged"]
+        assert abs(f["CL"] / fo["CL"] - 1) <= 1e-8, (prec, f, fo)
+        assert abs(f["CDp"] / fo["CDp"] - 1) <= 1e-8, (prec, f, fo)
+
+
+@pytest.mark.parametrize("pc", ["ilu0", "sgs"])
+def test_naca0012_solve_follows_oracle_in_the_same_ordering(torch_cuda, pc):
+    # the oracle on the cells sorted by the device's colours runs the same preconditioner
+    # sequentially: the two pseudo-time histories agree up to rounding
+    import test_oracle_solver as tos
+    raw = gmsh.read_gmsh2(golden("naca0012luo.msh"))
+    colour, _ = fvx.Preconditioner(fvx.Mesh(raw), pc).ordering()
+    perm = np.lexsort((np.arange(raw.nelem), colour))
+    rawp = raw.permuted(perm)
+    m = po.Mesh(rawp)
+    d1 = po.Disc(m, po.make_config(Minf=0.8, aoa_deg=1.25, bcs=tos.NACA_BCS, flux="HLLC",
+                                   gradient="NONE", order2=False))
+    d2 = po.Disc(m, po.make_config(Minf=0.8, aoa_deg=1.25, bcs=tos.NACA_BCS, flux="HLLC",
+                                   gradient="LEASTSQUARES", recon="VANALBADA",
+                                   limiter_param=20.0))
+    uo = np.tile(d1.freestream(), (raw.nelem, 1))
+    uo, o1 = d1.steady_solve(uo, cflinit=50.0, cflfin=1000.0, tol=1e-1, maxiter=20,
+                             fullupdate=0, min_relax=0.2, pc=pc)
+    uo, o2 = d2.steady_solve(uo, cflinit=500.0, cflfin=5000.0, tol=1e-6, maxiter=400,
+                             fullupdate=0, min_relax=0.2, pc=pc)
+    fo = d2.functionals(uo, 2)
+    u, s1, s2, f = _naca(pc=pc, tol=1e-6, maxiter=400)
+    assert s1["steps"] == o1["steps"] and s2["steps"] == o2["steps"], (s1, s2, o1, o2)
+    assert abs(s2["lin_iters"] - o2["lin_iters"]) <= 0.02 * o2["lin_iters"]
+    assert abs(f["CL"] / fo["CL"] - 1) <= 1e-7 and abs(f["CDp"] / fo["CDp"] - 1) <= 1e-7
+    ug = u.cpu().numpy()[perm]
+    assert np.abs(ug - uo).max() <= 1e-6 * np.abs(uo).max()
+
+
+def test_matrix_free_and_assembled_take_equal_steps(torch_cuda):
+    # tests/solvers/testmatrixfree.cpp:65 with tests/solvers/matfree.ctrl / matfree.solverc:
+    # first-order HLLC cylinder, M 0.38, full update, CFL 50..3000, tol 1e-8, 100 steps, GMRES
+    # rtol 1e-2 (the shipped cylinder mesh stands in for the generated 2dcylquad3.msh)
+    import torch
+    raw = gmsh.read_gmsh2(golden("2dcylinder2.msh"))
+    gm = fvx.Mesh(raw.permuted(fvx.rcm_permutation(raw)))
+    d = fvx.FlowFV(gm, fvx.physics_cfg(Minf=0.38, bcs=CYL_BCS),
+                   fvx.numerics_cfg(flux="HLLC", gradient="NONE", order2=False))
+    steps = []
+    for mf in (False, True):
+        u = torch.tensor(np.tile(d.freestream(), (raw.nelem, 1)), device="cuda")
+        st = d.steady_solve(u, cflinit=50.0, cflfin=3000.0, tol=1e-8, maxiter=100,
+                            fullupdate=True, pc="ilu0", krylov_rtol=1e-2, matrix_free=mf,
+                            mf_eps=1e-6)
+        steps.append(st["steps"])
+    assert steps[0] == steps[1], steps
