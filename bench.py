@@ -98,7 +98,8 @@ def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {a.gpus} is not supported "
-              "(multi-GPU sharding is described in DESIGN.md, not run)", file=sys.stderr)
+              "(the engine runs on a single GPU; domain sharding is not implemented)",
+              file=sys.stderr)
         sys.exit(2)
     import torch
     from fvens_amd import fvx

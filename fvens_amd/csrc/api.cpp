@@ -205,6 +205,14 @@ fvx_status fvx_pc_destroy(fvx_pc p)
 	return guarded([&] { delete p; });
 }
 
+fvx_status fvx_pc_set_sweeps(fvx_pc p, int sweeps)
+{
+	return guarded([&] {
+		require(p != nullptr, "null handle");
+		fvx::pc_set_sweeps(*p, sweeps);
+	});
+}
+
 fvx_status fvx_pc_setup(fvx_pc p, fvx_matrix A, void* stream)
 {
 	return guarded([&] {

@@ -134,6 +134,9 @@ struct fvx_matrix_s {
 struct fvx_pc_s {
 	const fvx_mesh_s* m = nullptr;
 	int type = FVX_PC_NONE, prec = FVX_PREC_F64;
+	int sweeps = 1;                        // stationary iterations per application
+	const fvx_matrix_s* A = nullptr;       // matrix of the last set-up
+	fvx::DBuf<double> t1, t2;              // residual and correction for sweeps > 1
 	int ncol = 0;
 	std::vector<int> colour, order, coff;
 	fvx::DBuf<int> d_colour, d_order;
@@ -194,6 +197,7 @@ void launch_update_explicit(int n, double* u, const double* r, const double* dt,
 // host side of the implicit solver (solver.cpp)
 void matrix_init(fvx_matrix_s& A, const fvx_mesh_s* m, int prec);
 void pc_init(fvx_pc_s& p, const fvx_mesh_s* m, int type, int prec);
+void pc_set_sweeps(fvx_pc_s& p, int sweeps);
 void matrix_export_bsr(const fvx_matrix_s& A, int* rowptr, int* colind, double* vals);
 int matrix_bsr_nnzb(const fvx_mesh_s& m);
 void matfree_apply(const fvx_disc_s& d, const double* u, const double* ru, const double* dtm,

@@ -137,6 +137,40 @@ void k_spmv(int n, const VT* __restrict__ vals, const int* __restrict__ nbr,
 	y[4*c + q] = acc;
 }
 
+/// y = b - A x
+template <class VT>
+__global__ __launch_bounds__(BLOCK)
+void k_spmv_residual(int n, const VT* __restrict__ vals, const int* __restrict__ nbr,
+                     const double* __restrict__ x, const double* __restrict__ b,
+                     double* __restrict__ y)
+{
+	const int q = threadIdx.x & 3;
+	const int c = xcd_block()*CELLS_PER_BLOCK + (threadIdx.x >> 2);
+	if(c >= n) return;
+	const int4 nb = load_nbr(nbr, c);
+	const int nbs[4] = { nb.x, nb.y, nb.z, nb.w };
+	const VT* blk = vals + (size_t)c*80 + 4*q;
+	double a[4], xv[4];
+	load_row<VT>(blk, a);
+	load4(x + 4*c, xv);
+	double acc = dot4(a, xv);
+	for(int k = 0; k < 4; k++) {
+		if(nbs[k] < 0) continue;
+		load_row<VT>(blk + 16*(1 + k), a);
+		load4(x + 4*nbs[k], xv);
+		acc += dot4(a, xv);
+	}
+	y[4*c + q] = b[4*c + q] - acc;
+}
+
+/// y += x
+__global__ __launch_bounds__(BLOCK)
+void k_axpy1(long N, const double* __restrict__ x, double* __restrict__ y)
+{
+	const long stride = (long)gridDim.x*blockDim.x;
+	for(long e = (long)blockIdx.x*blockDim.x + threadIdx.x; e < N; e += stride) y[e] += x[e];
+}
+
 /// A_ii += dtm_i I with dtm_i = area_i/(cfl dt_i).
 template <class VT>
 __global__ __launch_bounds__(BLOCK)
@@ -188,10 +222,10 @@ void k_jacobi_apply(int n, const VTF* __restrict__ fac, const double* __restrict
 	z[4*c + q] = dot4(a, rv);
 }
 
-/// One colour of a symmetric block Gauss-Seidel sweep (PETSc SOR with omega = 1, one local
-/// symmetric iteration from zero). Forward: z_i = D_i^-1 (r_i - sum_{lower j} A_ij z_j);
-/// backward: z_i = D_i^-1 (r_i - sum_{j != i} A_ij z_j).
-template <class VTF, bool BACK>
+/// One colour of a block Gauss-Seidel pass (PETSc SOR with omega = 1):
+/// z_i = D_i^-1 (r_i - sum_{j != i} A_ij z_j) with the current z (ALL), or with the lower
+/// colours only (the first forward pass from z = 0, where the other colours hold no value yet).
+template <class VTF, bool ALL>
 __global__ __launch_bounds__(BLOCK)
 void k_sgs_colour(int cnt, const int* __restrict__ list, const VTF* __restrict__ fac,
                   const int* __restrict__ nbr, const unsigned char* __restrict__ lowmask,
@@ -208,7 +242,7 @@ void k_sgs_colour(int cnt, const int* __restrict__ list, const VTF* __restrict__
 	double t = r[4*i + q];
 	double a[4], zv[4];
 	for(int k = 0; k < 4; k++) {
-		if(nbs[k] < 0 || !(BACK || ((lm >> k) & 1))) continue;
+		if(nbs[k] < 0 || !(ALL || ((lm >> k) & 1))) continue;
 		load_row<VTF>(blk + 16*(1 + k), a);
 		load4(z + 4*nbs[k], zv);
 		t -= dot4(a, zv);
@@ -605,6 +639,7 @@ static void pc_setup_t(fvx_pc_s& p, const VTA* vals, VTF* fac, hipStream_t s)
 
 void launch_pc_setup(fvx_pc_s& p, const fvx_matrix_s& A, hipStream_t s)
 {
+	p.A = &A;
 	if(p.type == FVX_PC_NONE) return;
 	if(A.prec == FVX_PREC_F32) {
 		if(p.prec == FVX_PREC_F32) pc_setup_t(p, A.f32(), p.f32(), s);
@@ -615,9 +650,38 @@ void launch_pc_setup(fvx_pc_s& p, const fvx_matrix_s& A, hipStream_t s)
 	}
 }
 
+/// One forward and one backward pass of symmetric Gauss-Seidel over the colours; the first
+/// forward pass (from z = 0) reads lower colours only.
 template <class VTF>
-static void pc_apply_t(const fvx_pc_s& p, const VTF* fac, const double* r, double* z,
-                       hipStream_t s)
+static void sgs_passes(const fvx_pc_s& p, const VTF* fac, const double* r, double* z,
+                       bool first, hipStream_t s)
+{
+	const int* nbr = p.m->d_nbr.p;
+	for(int c = 0; c < p.ncol; c++) {
+		const int cnt = p.coff[c+1] - p.coff[c];
+		if(cnt == 0) continue;
+		const int* list = p.d_order.p + p.coff[c];
+		if(first)
+			hipLaunchKernelGGL((k_sgs_colour<VTF, false>), dim3(quad_blocks(cnt)), dim3(BLOCK), 0,
+			                   s, cnt, list, fac, nbr, p.d_lowmask.p, r, z);
+		else
+			hipLaunchKernelGGL((k_sgs_colour<VTF, true>), dim3(quad_blocks(cnt)), dim3(BLOCK), 0,
+			                   s, cnt, list, fac, nbr, p.d_lowmask.p, r, z);
+		FVX_HIP(hipGetLastError());
+	}
+	for(int c = p.ncol - 1; c >= 0; c--) {
+		const int cnt = p.coff[c+1] - p.coff[c];
+		if(cnt == 0) continue;
+		hipLaunchKernelGGL((k_sgs_colour<VTF, true>), dim3(quad_blocks(cnt)), dim3(BLOCK), 0, s,
+		                   cnt, p.d_order.p + p.coff[c], fac, nbr, p.d_lowmask.p, r, z);
+		FVX_HIP(hipGetLastError());
+	}
+}
+
+/// z = M^-1 r for Jacobi or ILU(0) (one application).
+template <class VTF>
+static void pc_once(const fvx_pc_s& p, const VTF* fac, const double* r, double* z,
+                    hipStream_t s)
 {
 	const int n = p.m->nelem;
 	const int* nbr = p.m->d_nbr.p;
@@ -630,25 +694,44 @@ static void pc_apply_t(const fvx_pc_s& p, const VTF* fac, const double* r, doubl
 	for(int c = 0; c < p.ncol; c++) {
 		const int cnt = p.coff[c+1] - p.coff[c];
 		if(cnt == 0) continue;
-		const int* list = p.d_order.p + p.coff[c];
-		if(p.type == FVX_PC_SGS)
-			hipLaunchKernelGGL((k_sgs_colour<VTF, false>), dim3(quad_blocks(cnt)), dim3(BLOCK), 0,
-			                   s, cnt, list, fac, nbr, p.d_lowmask.p, r, z);
-		else
-			hipLaunchKernelGGL(k_ilu_fwd<VTF>, dim3(quad_blocks(cnt)), dim3(BLOCK), 0, s, cnt,
-			                   list, fac, nbr, p.d_lowmask.p, r, z);
+		hipLaunchKernelGGL(k_ilu_fwd<VTF>, dim3(quad_blocks(cnt)), dim3(BLOCK), 0, s, cnt,
+		                   p.d_order.p + p.coff[c], fac, nbr, p.d_lowmask.p, r, z);
 		FVX_HIP(hipGetLastError());
 	}
 	for(int c = p.ncol - 1; c >= 0; c--) {
 		const int cnt = p.coff[c+1] - p.coff[c];
 		if(cnt == 0) continue;
-		const int* list = p.d_order.p + p.coff[c];
-		if(p.type == FVX_PC_SGS)
-			hipLaunchKernelGGL((k_sgs_colour<VTF, true>), dim3(quad_blocks(cnt)), dim3(BLOCK), 0,
-			                   s, cnt, list, fac, nbr, p.d_lowmask.p, r, z);
+		hipLaunchKernelGGL(k_ilu_bwd<VTF>, dim3(quad_blocks(cnt)), dim3(BLOCK), 0, s, cnt,
+		                   p.d_order.p + p.coff[c], fac, nbr, p.d_lowmask.p, z);
+		FVX_HIP(hipGetLastError());
+	}
+}
+
+/// `sweeps` steps of z <- z + M^-1 (r - A z) from z = 0 (PETSc -pc_sor_its). Symmetric
+/// Gauss-Seidel does them as in-place passes; Jacobi and ILU(0) form the residual explicitly.
+template <class VTF>
+static void pc_apply_t(const fvx_pc_s& p, const VTF* fac, const double* r, double* z,
+                       hipStream_t s)
+{
+	if(p.type == FVX_PC_SGS) {
+		for(int k = 0; k < p.sweeps; k++) sgs_passes(p, fac, r, z, k == 0, s);
+		return;
+	}
+	pc_once(p, fac, r, z, s);
+	if(p.sweeps <= 1) return;
+	const fvx_matrix_s& A = *p.A;
+	const int n = p.m->nelem;
+	const long N = 4L*n;
+	for(int k = 1; k < p.sweeps; k++) {
+		if(A.prec == FVX_PREC_F32)
+			hipLaunchKernelGGL(k_spmv_residual<float>, dim3(quad_blocks(n)), dim3(BLOCK), 0, s, n,
+			                   A.f32(), p.m->d_nbr.p, z, r, p.t1.p);
 		else
-			hipLaunchKernelGGL(k_ilu_bwd<VTF>, dim3(quad_blocks(cnt)), dim3(BLOCK), 0, s, cnt,
-			                   list, fac, nbr, p.d_lowmask.p, z);
+			hipLaunchKernelGGL(k_spmv_residual<double>, dim3(quad_blocks(n)), dim3(BLOCK), 0, s, n,
+			                   A.f64(), p.m->d_nbr.p, z, r, p.t1.p);
+		FVX_HIP(hipGetLastError());
+		pc_once(p, fac, p.t1.p, p.t2.p, s);
+		hipLaunchKernelGGL(k_axpy1, dim3(stream_blocks(N)), dim3(BLOCK), 0, s, N, p.t2.p, z);
 		FVX_HIP(hipGetLastError());
 	}
 }
@@ -660,6 +743,7 @@ void launch_pc_apply(const fvx_pc_s& p, const double* r, double* z, hipStream_t 
 		                       hipMemcpyDeviceToDevice, s));
 		return;
 	}
+	if(!p.A) throw Error(FVX_ERR_INVALID, "preconditioner applied before its set-up");
 	if(p.prec == FVX_PREC_F32) pc_apply_t(p, p.f32(), r, z, s);
 	else pc_apply_t(p, p.f64(), r, z, s);
 }

@@ -128,6 +128,16 @@ void pc_init(fvx_pc_s& p, const fvx_mesh_s* m, int type, int prec)
 	}
 }
 
+void pc_set_sweeps(fvx_pc_s& p, int sweeps)
+{
+	if(sweeps < 1 || sweeps > 64) throw Error(FVX_ERR_INVALID, "sweep count must be in [1, 64]");
+	p.sweeps = sweeps;
+	if(sweeps > 1 && p.type != FVX_PC_SGS && p.type != FVX_PC_NONE) {
+		const size_t N = 4*(size_t)p.m->nelem;
+		if(p.t1.n != N) { p.t1.alloc(N); p.t2.alloc(N); }
+	}
+}
+
 /* ---------------------------------------------------------------- Krylov --------------- */
 
 /// Device storage of one FGMRES cycle and the pinned host slots for its scalars.
@@ -303,6 +313,7 @@ void steady_solve(fvx_disc_s& d, const fvx_solver_cfg& c, double* u, fvx_solve_s
 			w.pctype = c.pc;
 			w.pcprec = c.pc_precision;
 		}
+		pc_set_sweeps(w.P, c.pc_sweeps > 0 ? c.pc_sweeps : 1);
 		w.kry.ensure(N, c.krylov_restart);
 	}
 	Krylov& K = w.kry;

@@ -279,7 +279,7 @@ class SolverCfg(C.Structure):
     _fields_ = [("cflinit", C.c_double), ("cflfin", C.c_double), ("tol", C.c_double),
                 ("maxiter", C.c_int), ("implicit", C.c_int), ("fullupdate", C.c_int),
                 ("min_relax", C.c_double), ("pc", C.c_int), ("pc_precision", C.c_int),
-                ("jac_precision", C.c_int), ("krylov_restart", C.c_int),
+                ("pc_sweeps", C.c_int), ("jac_precision", C.c_int), ("krylov_restart", C.c_int),
                 ("krylov_rtol", C.c_double), ("matrix_free", C.c_int), ("mf_eps", C.c_double),
                 ("fixed_norm", C.c_double)]
 
@@ -291,7 +291,7 @@ class SolveStats(C.Structure):
 
 
 def solver_cfg(cflinit=500.0, cflfin=5000.0, tol=1e-7, maxiter=170, implicit=True,
-               fullupdate=False, min_relax=0.1, pc="ilu0", pc_precision="f64",
+               fullupdate=False, min_relax=0.2, pc="sgs", pc_precision="f64", pc_sweeps=1,
                jac_precision="f64", krylov_restart=30, krylov_rtol=1e-1, matrix_free=False,
                mf_eps=1e-7, fixed_norm=0.0):
     """Defaults: testcases/naca0012/transonic-sanity-test-muscl.ctrl main solve with the
@@ -300,6 +300,7 @@ def solver_cfg(cflinit=500.0, cflfin=5000.0, tol=1e-7, maxiter=170, implicit=Tru
     c.cflinit, c.cflfin, c.tol, c.maxiter = cflinit, cflfin, tol, maxiter
     c.implicit, c.fullupdate, c.min_relax = int(implicit), int(fullupdate), min_relax
     c.pc, c.pc_precision, c.jac_precision = PC[pc], PREC[pc_precision], PREC[jac_precision]
+    c.pc_sweeps = pc_sweeps
     c.krylov_restart, c.krylov_rtol = krylov_restart, krylov_rtol
     c.matrix_free, c.mf_eps, c.fixed_norm = int(matrix_free), mf_eps, fixed_norm
     return c
@@ -340,19 +341,22 @@ class Matrix:
 class Preconditioner:
     """Device preconditioner (none / block Jacobi / multicolour SGS / multicolour BILU(0))."""
 
-    def __init__(self, mesh: Mesh, kind="ilu0", precision="f64"):
+    def __init__(self, mesh: Mesh, kind="ilu0", precision="f64", sweeps=1):
         self.mesh = mesh
         h = C.c_void_p()
         check(lib().fvx_pc_create(mesh.h, PC[kind], PREC[precision], C.byref(h)))
         self.h = h
+        check(lib().fvx_pc_set_sweeps(h, int(sweeps)))
+        self._A = None
+
+    def setup(self, A: "Matrix"):
+        check(lib().fvx_pc_setup(self.h, A.h, _stream()))
+        self._A = A          # the sweeps read A: keep it alive with the preconditioner
 
     def __del__(self):
         if getattr(self, "h", None) is not None and _lib is not None:
             _lib.fvx_pc_destroy(self.h)
             self.h = None
-
-    def setup(self, A: Matrix):
-        check(lib().fvx_pc_setup(self.h, A.h, _stream()))
 
     def apply(self, r, z):
         n = self.mesh.nelem

@@ -165,7 +165,11 @@ fvx_status fvx_matrix_export_bsr(fvx_matrix A, int* rowptr, int* colind, double*
 /* PCCreate/PCSetType for a matrix of this mesh; precision = storage of the factors. */
 fvx_status fvx_pc_create(fvx_mesh m, int type, int precision, fvx_pc* out);
 fvx_status fvx_pc_destroy(fvx_pc p);
-/* PCSetUp: factorises / inverts from the current values of A. */
+/* Number of stationary iterations z <- z + M^-1 (r - A z) from z = 0 per application
+ * (PETSc -pc_sor_its; for SGS, symmetric forward + backward sweeps). Default 1, at most 64. */
+fvx_status fvx_pc_set_sweeps(fvx_pc p, int sweeps);
+/* PCSetUp: factorises / inverts from the current values of A; later applications use A for
+ * the sweeps, so A must stay alive and unchanged until the next set-up. */
 fvx_status fvx_pc_setup(fvx_pc p, fvx_matrix A, void* stream);
 /* PCApply: z = M^-1 r (device vectors [ncell][4]). */
 fvx_status fvx_pc_apply(fvx_pc p, const double* r, double* z, void* stream);
@@ -195,6 +199,7 @@ typedef struct {
 	double min_relax;         /* minimum local relaxation factor */
 	int pc;                   /* FVX_PC_* */
 	int pc_precision;         /* FVX_PREC_* of the preconditioner factors */
+	int pc_sweeps;            /* stationary iterations per application (0 or 1: one) */
 	int jac_precision;        /* FVX_PREC_* of the assembled Jacobian */
 	int krylov_restart;       /* -ksp_max_it (one FGMRES cycle) */
 	double krylov_rtol;       /* -ksp_rtol */

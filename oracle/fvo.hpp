@@ -195,6 +195,7 @@ struct SolverConfig {
 	int matrix_free;         ///< 1: matrix-free operator with FD step eps, M as preconditioner
 	double mf_eps;
 	double fixed_norm;       ///< >0: use this normaliser instead of the first-step norm
+	int pc_sweeps;           ///< preconditioner applications per Krylov step (>= 1)
 };
 
 struct SolveStats {
@@ -209,6 +210,8 @@ SolveStats steady_solve(const Disc& d, const SolverConfig& sc, double* u);
 /// Block ILU(0) / SGS / Jacobi preconditioner on a BSR matrix (host reference semantics).
 struct BlockPC {
 	int type = PC_NONE;
+	int sweeps = 1;                 // stationary iterations z += M^-1 (r - A z) from z = 0
+	BSR A;                          // the matrix (for sweeps > 1)
 	BSR F;                          // factored copy (ILU0) or the matrix (SGS)
 	std::vector<double> dinv;       // inverted diagonal blocks
 };

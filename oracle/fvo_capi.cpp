@@ -183,7 +183,7 @@ int fvo_steady_solve(void* dp, const SolverConfig* sc, double* u, double* stats)
 }
 
 int fvo_pc_apply_bsr(int n, const int* rowptr, const int* colind, const double* vals, int pctype,
-                     const double* r, double* z)
+                     int sweeps, const double* r, double* z)
 {
 	FVO_TRY({
 		BSR A;
@@ -195,6 +195,7 @@ int fvo_pc_apply_bsr(int n, const int* rowptr, const int* colind, const double* 
 		for(int i = 0; i < n; i++)
 			for(int k = rowptr[i]; k < rowptr[i+1]; k++) if(colind[k] == i) A.diag[i] = k;
 		BlockPC p;
+		p.sweeps = sweeps > 1 ? sweeps : 1;
 		pc_setup(p, A, pctype);
 		pc_apply(p, r, z);
 	})

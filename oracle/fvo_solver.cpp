@@ -83,6 +83,7 @@ void pc_setup(BlockPC& p, const BSR& A, int type)
 {
 	p.type = type;
 	p.F = A;
+	if(p.sweeps > 1) p.A = A;
 	const int n = A.n;
 	p.dinv.assign((size_t)n*16, 0.0);
 	if(type == PC_ILU0) {
@@ -115,7 +116,26 @@ void pc_setup(BlockPC& p, const BSR& A, int type)
 	}
 }
 
+static void pc_apply_once(const BlockPC& p, const double* r, double* z);
+
+/* PETSc -pc_sor_its semantics for every type: `sweeps` steps of the stationary iteration
+ * z <- z + M^-1 (r - A z) from z = 0. For symmetric Gauss-Seidel this equals `sweeps` in-place
+ * forward + backward sweeps. */
 void pc_apply(const BlockPC& p, const double* r, double* z)
+{
+	pc_apply_once(p, r, z);
+	if(p.sweeps <= 1 || p.type == PC_NONE) return;
+	const size_t N = (size_t)p.F.n*4;
+	std::vector<double> t(N), dz(N);
+	for(int s = 1; s < p.sweeps; s++) {
+		bsr_matvec(p.A, z, t.data());
+		for(size_t i = 0; i < N; i++) t[i] = r[i] - t[i];
+		pc_apply_once(p, t.data(), dz.data());
+		for(size_t i = 0; i < N; i++) z[i] += dz[i];
+	}
+}
+
+static void pc_apply_once(const BlockPC& p, const double* r, double* z)
 {
 	const BSR& F = p.F;
 	const int n = F.n;
@@ -254,6 +274,7 @@ SolveStats steady_solve(const Disc& d, const SolverConfig& sc, double* u)
 	std::vector<double> r(N), du(N), dtm(ne), aux(N), yg(N);
 	BSR M;
 	BlockPC pc;
+	pc.sweeps = sc.pc_sweeps > 1 ? sc.pc_sweeps : 1;
 	double curCFL = 0, resi = 1.0, resiold = 1.0, initres = 1.0;
 	if(sc.fixed_norm > 0) initres = sc.fixed_norm;
 	int step = 0;

@@ -17,7 +17,7 @@
  *  Jacobian:        src/spatial/aspatial.cpp:242-340, src/spatial/flow_spatial.cpp:397-446,818-875
  *  functionals:     src/spatial/aoutput.cpp:28-62, src/spatial/flow_spatial.cpp:95-310
  *
- * Deliberate definitions where the reference is undefined (recorded in DESIGN.md):
+ * Deliberate definitions where the reference is undefined:
  *  - Venkatakrishnan / Barth-Jespersen read the ghost primitive state for boundary neighbours
  *    (the reference reads past the end of its cell array, limitedlinearreconstruction.cpp:134,229);
  *  - the limiter parameter K / lambda is an explicit configuration value (never parsed by the

@@ -39,7 +39,7 @@ class SolverConfig(C.Structure):
                 ("maxiter", C.c_int), ("implicit", C.c_int), ("fullupdate", C.c_int),
                 ("min_relax", C.c_double), ("pc", C.c_int), ("krylov_restart", C.c_int),
                 ("krylov_rtol", C.c_double), ("matrix_free", C.c_int), ("mf_eps", C.c_double),
-                ("fixed_norm", C.c_double)]
+                ("fixed_norm", C.c_double), ("pc_sweeps", C.c_int)]
 
 
 def lib():
@@ -238,7 +238,8 @@ class Disc:
         sc = SolverConfig()
         defaults = dict(cflinit=50.0, cflfin=500.0, tol=1e-10, maxiter=500, implicit=1,
                         fullupdate=1, min_relax=0.2, pc="ilu0", krylov_restart=30,
-                        krylov_rtol=1e-1, matrix_free=0, mf_eps=1e-7, fixed_norm=0.0)
+                        krylov_rtol=1e-1, matrix_free=0, mf_eps=1e-7, fixed_norm=0.0,
+                        pc_sweeps=1)
         defaults.update(kw)
         for k, v in defaults.items():
             setattr(sc, k, PC[v] if k == "pc" else v)
@@ -268,13 +269,13 @@ def flux_jacobian(cfg, name, ul, ur, n):
     return dl, dr
 
 
-def pc_apply_bsr(rowptr, colind, vals, pc, r):
+def pc_apply_bsr(rowptr, colind, vals, pc, r, sweeps=1):
     rowptr = np.ascontiguousarray(rowptr, np.int32)
     colind = np.ascontiguousarray(colind, np.int32)
     vals = np.ascontiguousarray(vals, np.float64)
     r = np.ascontiguousarray(r, np.float64)
     z = np.zeros_like(r)
     _check(lib().fvo_pc_apply_bsr(len(rowptr) - 1, _p(rowptr, C.c_int), _p(colind, C.c_int),
-                                  _p(vals, C.c_double), PC[pc], _p(r, C.c_double),
+                                  _p(vals, C.c_double), PC[pc], int(sweeps), _p(r, C.c_double),
                                   _p(z, C.c_double)))
     return z

@@ -152,6 +152,16 @@ def _perm_bsr(rp, ci, v, perm):
 @pytest.mark.parametrize("kind", ["jacobi", "sgs", "ilu0"])
 @pytest.mark.parametrize("prec", ["f64", "f32"])
 def test_preconditioner_parity(torch_cuda, meshname, kind, prec):
+    _pc_compare(meshname, kind, prec, 1)
+
+
+@pytest.mark.parametrize("kind,sweeps", [("sgs", 4), ("ilu0", 3), ("jacobi", 3)])
+def test_preconditioner_sweeps_parity(torch_cuda, kind, sweeps):
+    # several stationary iterations per application (PETSc -pc_sor_its)
+    _pc_compare("2dcylinderhybrid.msh", kind, "f64", sweeps)
+
+
+def _pc_compare(meshname, kind, prec, sweeps):
     import torch
     raw = _raw(meshname)
     bcs = BUMP_BCS if meshname == "bump" else CYL_BCS
@@ -164,7 +174,7 @@ def test_preconditioner_parity(torch_cuda, meshname, kind, prec):
     dtm = torch.zeros(n, dtype=torch.float64, device="cuda")
     gd.compute_residual(ut, r, True, dt)
     gd.add_pseudotime(A, dt, 50.0, dtm)
-    P = fvx.Preconditioner(gm, kind, prec)
+    P = fvx.Preconditioner(gm, kind, prec, sweeps=sweeps)
     P.setup(A)
     rhs = np.random.default_rng(3).standard_normal((n, 4))
     z = torch.zeros((n, 4), dtype=torch.float64, device="cuda")
@@ -179,7 +189,7 @@ def test_preconditioner_parity(torch_cuda, meshname, kind, prec):
         assert ncol <= 5
     perm = np.lexsort((np.arange(n), colour))
     prp, pci, pv = _perm_bsr(rp, ci, v, perm)
-    zp = po.pc_apply_bsr(prp, pci, pv, kind, rhs[perm].ravel()).reshape(-1, 4)
+    zp = po.pc_apply_bsr(prp, pci, pv, kind, rhs[perm].ravel(), sweeps).reshape(-1, 4)
     zo = np.empty_like(zp); zo[perm] = zp
     tol = 1e-11 if prec == "f64" else 2e-5
     err = np.abs(zg - zo).max() / np.abs(zo).max()

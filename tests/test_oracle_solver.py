@@ -254,3 +254,29 @@ def test_naca0012_reference_regression():
         CL, CDp = (float(t) for t in fh.read().split()[:2])
     assert abs(f["CL"] - CL) / abs(CL) <= 1e-6
     assert abs(f["CDp"] - CDp) / abs(CDp) <= 1e-8
+
+
+def test_preconditioner_sweeps_are_stationary_iterations():
+    # sweeps = k: z_k = z_{k-1} + M^-1 (r - A z_{k-1}) from z_0 = 0 with the dense splitting
+    rng = np.random.default_rng(12)
+    n = 10
+    edges = [(0, 1), (1, 2), (2, 3), (0, 4), (4, 5), (5, 2), (6, 7), (7, 8), (8, 9), (3, 9), (1, 6)]
+    rp, ci, vals = _random_bsr(rng, n, edges)
+    A = _dense(rp, ci, vals, n)
+    D = np.zeros_like(A)
+    for i in range(n):
+        D[4 * i:4 * i + 4, 4 * i:4 * i + 4] = A[4 * i:4 * i + 4, 4 * i:4 * i + 4]
+    M = {"sgs": (D + np.tril(A - D)) @ np.linalg.inv(D) @ (D + np.triu(A - D)), "jacobi": D}
+    r = rng.standard_normal(4 * n)
+    for kind, Mk in M.items():
+        z = np.zeros(4 * n)
+        for k in range(1, 5):
+            z = z + np.linalg.solve(Mk, r - A @ z)
+            assert np.allclose(po.pc_apply_bsr(rp, ci, vals, kind, r, k), z, rtol=1e-11,
+                               atol=1e-11), (kind, k)
+    # ILU(0) on a chain is exact: further sweeps leave the solution unchanged
+    rp, ci, vals = _random_bsr(rng, 8, [(i, i + 1) for i in range(7)])
+    A = _dense(rp, ci, vals, 8)
+    r = rng.standard_normal(32)
+    assert np.allclose(po.pc_apply_bsr(rp, ci, vals, "ilu0", r, 3), np.linalg.solve(A, r),
+                       rtol=1e-11, atol=1e-11)
