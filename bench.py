@@ -6,7 +6,8 @@ K=5, M0.8, alpha 1.25 deg) on the refined 2048x512 NACA 4-digit (t=0.12) O-mesh 
 FlowFV::compute_residual (gradients + limiter pass, face-flux / gather pass with local time
 steps) over a synthetic, seeded perturbed free-stream state resident in HBM.
 
-Prints one JSON line. The CPU baseline is this repository's CPU oracle in its
+Prints one JSON line. It also times pseudo-time solves to a residual tolerance (the second
+half of the metric, under "time_to_tol"). The CPU baseline is this repository's CPU oracle in its
 reference-structured OpenMP mode (face loops with atomic scatters), timed on a bounded sample
 of the same workload on the host cores.
 """
@@ -37,6 +38,8 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=12.0,
                    help="bounded CPU-baseline sample (seconds of CPU work)")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-solve", action="store_true",
+                   help="skip the time-to-tolerance pseudo-time solves")
     p.add_argument("--calibrate-copy", action="store_true",
                    help="run only a plain device-to-device copy of known size (the reference "
                         "kernel for scaling HBM counter readings)")
@@ -72,6 +75,92 @@ def build_case():
                             limiter_param=5.0)
     u = states.perturbed_freestream(raw, Minf=0.8, aoa_deg=1.25, L=1.0)
     return raw, bcs, pcfg, ncfg, np.ascontiguousarray(u)
+
+
+BUMP_BCS = [(2, "slipwall"), (3, "inflowoutflow"), (4, "inflowoutflow")]
+AIRFOIL_BCS = [(2, "slipwall"), (4, "farfield")]
+
+
+def _solve_case(name):
+    """Mesh, physics and (first-order, second-order) numerics of a time-to-tolerance case."""
+    from fvens_amd import fvx, meshgen
+    if name == "bump":
+        raw = meshgen.mesh_k1()
+        phys = dict(Minf=0.5, aoa_deg=0.0, bcs=BUMP_BCS)
+        n1 = dict(flux="ROE", gradient="NONE", order2=False)
+        n2 = dict(flux="ROE", gradient="GREENGAUSS", recon="NONE")
+        # tests/inv-gaussianbump/base.ctrl: starter CFL 25..500, main CFL 250..2000
+        ramps = ((25.0, 500.0), (250.0, 2000.0))
+    else:
+        raw = meshgen.mesh_k2()
+        phys = dict(Minf=0.5, aoa_deg=1.25, bcs=AIRFOIL_BCS)
+        n1 = dict(flux="HLLC", gradient="NONE", order2=False)
+        n2 = dict(flux="HLLC", gradient="LEASTSQUARES", recon="VENKATAKRISHNAN",
+                  limiter_param=5.0)
+        ramps = ((1000.0, 1000.0), (1000.0, 10000.0))
+    raw = raw.permuted(fvx.rcm_permutation(raw))
+    return raw, phys, n1, n2, ramps
+
+
+def gpu_time_to_tol(name, tol, precision, sweeps=16):
+    """Pseudo-time solve from free stream (first-order starter, then the main solve) until
+    sqrt(sum r_E^2 area) <= tol x the same norm at the free-stream state."""
+    import numpy as np
+    import torch
+    from fvens_amd import fvx
+    raw, phys, n1, n2, ramps = _solve_case(name)
+    mesh = fvx.Mesh(raw)
+    pc = fvx.physics_cfg(**phys)
+    d1 = fvx.FlowFV(mesh, pc, fvx.numerics_cfg(**n1))
+    d2 = fvx.FlowFV(mesh, pc, fvx.numerics_cfg(**n2))
+    uinf = torch.tensor(np.tile(d1.freestream(), (raw.nelem, 1)), device="cuda")
+    r = torch.zeros_like(uinf)
+    dt = torch.zeros(raw.nelem, dtype=torch.float64, device="cuda")
+    d2.compute_residual(uinf, r, True, dt)
+    _, _, area, _ = mesh.faces()
+    norm0 = float(np.sqrt((r[:, 3].cpu().numpy() ** 2 * area).sum()))
+    kw = dict(min_relax=0.2, pc="sgs", pc_sweeps=sweeps, pc_precision=precision,
+              jac_precision=precision, raise_on_failure=False)
+    u = uinf.clone()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s1 = d1.steady_solve(u, cflinit=ramps[0][0], cflfin=ramps[0][1], tol=1e-1 if name == "bump"
+                         else 1e-6, maxiter=300, **kw)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    s2 = d2.steady_solve(u, cflinit=ramps[1][0], cflfin=ramps[1][1], tol=tol, maxiter=3000,
+                         fixed_norm=norm0, **kw)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    f = d2.functionals(u, 2)
+    return dict(seconds=round(t2 - t0, 4), main_seconds=round(t2 - t1, 4),
+                starter_steps=s1["steps"], steps=s2["steps"], lin_iters=s2["lin_iters"],
+                final_rel=s2["final_rel"], converged=bool(s2["converged"]),
+                CL=f["CL"], CDp=f["CDp"], precision=precision, pc=f"multicolour SGS x{sweeps}")
+
+
+def cpu_time_to_tol(name, tol):
+    """The oracle's FP64 implicit solve of the same case: natural-order block ILU(0), the
+    reference's bjacobi + ILU on one rank (tests/inv-gaussianbump/ibump.solverc)."""
+    import numpy as np
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po
+    raw, phys, n1, n2, ramps = _solve_case(name)
+    m = po.Mesh(raw)
+    d1 = po.Disc(m, po.make_config(**phys, **n1))
+    d2 = po.Disc(m, po.make_config(**phys, **n2))
+    u = np.tile(d1.freestream(), (raw.nelem, 1))
+    r, _ = d2.residual(u)
+    norm0 = float(np.sqrt((r[:, 3] ** 2 * m.area).sum()))
+    t0 = time.perf_counter()
+    u, s1 = d1.steady_solve(u, cflinit=ramps[0][0], cflfin=ramps[0][1], tol=1e-1, maxiter=300,
+                            fullupdate=0, min_relax=0.2, pc="ilu0")
+    u, s2 = d2.steady_solve(u, cflinit=ramps[1][0], cflfin=ramps[1][1], tol=tol, maxiter=3000,
+                            fullupdate=0, min_relax=0.2, pc="ilu0", fixed_norm=norm0)
+    f = d2.functionals(u, 2)
+    return dict(seconds=round(time.perf_counter() - t0, 3), steps=s2["steps"],
+                lin_iters=s2["lin_iters"], final_rel=s2["rel"], converged=bool(s2["converged"]),
+                CL=f["CL"], CDp=f["CDp"], threads=1, pc="natural-order block ILU(0)")
 
 
 def cpu_baseline(raw, bcs, u, budget_s):
@@ -162,6 +251,35 @@ def main():
     }
     if not a.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(raw, bcs, u_host, a.cpu_seconds)
+    if not a.no_solve:
+        # second half of the metric: time to 1e-10 of the free-stream residual norm
+        del disc, mesh, u, r, dt
+        bump64 = gpu_time_to_tol("bump", 1e-10, "f64")
+        bump32 = gpu_time_to_tol("bump", 1e-10, "f32")
+        air64 = gpu_time_to_tol("airfoil", 1e-8, "f64")
+        air32 = gpu_time_to_tol("airfoil", 1e-8, "f32")
+        tt = {
+            "criterion": "sqrt(sum r_energy^2 area) <= tol x the same norm at the free stream",
+            "bump_1e-10": {
+                "case": "Gaussian bump 96x32 hybrid, 4,608 cells, M0.5, Roe, Green-Gauss, "
+                        "unlimited; first-order starter then main solve",
+                "gpu_fp64": bump64, "gpu_fp32_jacobian_and_pc": bump32,
+                "fp32_speedup": round(bump64["seconds"] / bump32["seconds"], 3),
+                "warm_start": None},
+            "airfoil_1e-8": {
+                "case": "NACA 4-digit O-mesh 512x128, 98,304 cells, M0.5, alpha 1.25, HLLC, WLS, "
+                        "Venkatakrishnan K=5 (the limiter stalls the residual near 3e-9, so "
+                        "1e-8 is the airfoil target)",
+                "gpu_fp64": air64, "gpu_fp32_jacobian_and_pc": air32,
+                "fp32_speedup": round(air64["seconds"] / air32["seconds"], 3),
+                "CL_CDp_fp32_vs_fp64": [abs(air32["CL"] / air64["CL"] - 1),
+                                        abs(air32["CDp"] / air64["CDp"] - 1)]},
+        }
+        if not a.no_cpu_baseline:
+            tt["bump_1e-10"]["cpu_oracle_fp64"] = cpu_time_to_tol("bump", 1e-10)
+            tt["bump_1e-10"]["gpu_vs_cpu"] = round(
+                tt["bump_1e-10"]["cpu_oracle_fp64"]["seconds"] / bump64["seconds"], 2)
+        out["time_to_tol"] = tt
     print(json.dumps(out), flush=True)
 
 

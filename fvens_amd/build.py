@@ -60,8 +60,17 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
     if p.returncode != 0:
         raise RuntimeError(f"link failed: {' '.join(cmd)}\n{p.stdout}\n{p.stderr}")
     os.replace(tmp, LIB)
+    # C++ case driver over the host adapter (host code only, links libfvx.so)
+    drv = os.path.join(HERE, "host", "fvx_case")
+    cmd = [HIPCC, "-std=c++17", "-O2", "-Wall", os.path.join(HERE, "host", "fvx_case.cpp"),
+           "-I", os.path.join(HERE, "..", "include"), "-L", HERE, "-lfvx",
+           "-Wl,-rpath,$ORIGIN/..", "-o", drv + ".tmp"]
+    p = subprocess.run(cmd, capture_output=True, text=True)
+    if p.returncode != 0:
+        raise RuntimeError(f"driver build failed: {' '.join(cmd)}\n{p.stdout}\n{p.stderr}")
+    os.replace(drv + ".tmp", drv)
     if verbose:
-        print(f"built {LIB}", file=sys.stderr)
+        print(f"built {LIB} and {drv}", file=sys.stderr)
     return LIB
 
 
