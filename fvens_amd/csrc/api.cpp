@@ -1,6 +1,7 @@
 /* api.cpp -- the extern "C" boundary of libfvx (declared in include/fvx.h). Converts internal
  * exceptions to status codes and keeps the last error text per thread. */
 #include "fvx_internal.hpp"
+#include <cmath>
 #include <cstring>
 #include <memory>
 
@@ -170,7 +171,8 @@ fvx_status fvx_matrix_apply(fvx_matrix A, const double* x, double* y, void* stre
 {
 	return guarded([&] {
 		require(A && x && y, "null argument");
-		fvx::launch_spmv(*A, x, y, static_cast<hipStream_t>(stream));
+		const long N = 4L*A->m->nelem;
+		fvx::launch_spmv(*A, x, N, y, N, fvx::Inst(), static_cast<hipStream_t>(stream));
 	});
 }
 
@@ -218,7 +220,8 @@ fvx_status fvx_pc_setup(fvx_pc p, fvx_matrix A, void* stream)
 	return guarded([&] {
 		require(p && A, "null argument");
 		require(p->m == A->m, "preconditioner and matrix belong to different meshes");
-		fvx::launch_pc_setup(*p, *A, static_cast<hipStream_t>(stream));
+		require(A->ninst >= p->ninst, "matrix holds fewer instances than the preconditioner");
+		fvx::launch_pc_setup(*p, *A, fvx::Inst(), static_cast<hipStream_t>(stream));
 	});
 }
 
@@ -226,7 +229,8 @@ fvx_status fvx_pc_apply(fvx_pc p, const double* r, double* z, void* stream)
 {
 	return guarded([&] {
 		require(p && r && z, "null argument");
-		fvx::launch_pc_apply(*p, r, z, static_cast<hipStream_t>(stream));
+		const long N = 4L*p->m->nelem;
+		fvx::launch_pc_apply(*p, r, N, z, N, fvx::Inst(), static_cast<hipStream_t>(stream));
 	});
 }
 
@@ -262,9 +266,9 @@ fvx_status fvx_matfree_apply(fvx_disc d, const double* u, const double* r_u, con
 		fvx::DBuf<double> aux, raux, part;
 		aux.alloc(N);
 		raux.alloc(N);
-		part.alloc((size_t)fvx::reduction_blocks()*fvx::max_dots() + 1);
+		part.alloc((size_t)fvx::partial_stride() + 1);
 		fvx::matfree_apply(*d, u, r_u, dtm, x, y, eps, aux.p, raux.p, part.p,
-		                   part.p + (size_t)fvx::reduction_blocks()*fvx::max_dots(), s);
+		                   part.p + (size_t)fvx::partial_stride(), s);
 		FVX_HIP(hipStreamSynchronize(s));
 	});
 }
@@ -277,6 +281,168 @@ fvx_status fvx_steady_solve(fvx_disc d, const fvx_solver_cfg* cfg, double* u,
 	return guarded([&] {
 		require(d && cfg && u, "null argument");
 		fvx::steady_solve(*d, *cfg, u, *st, static_cast<hipStream_t>(stream));
+	});
+}
+
+/* ------------------------------------------------------------- batched solve ----------- */
+
+fvx_status fvx_batch_create(fvx_disc d, int ninst, const double* mach, const double* aoa,
+                            fvx_batch* out)
+{
+	return guarded([&] {
+		require(d && mach && aoa && out, "null argument");
+		require(ninst >= 1 && ninst <= 4096, "instance count must be in [1, 4096]");
+		std::unique_ptr<fvx_batch_s> b(new fvx_batch_s);
+		b->d = d;
+		b->ninst = ninst;
+		std::vector<int> all(ninst);
+		for(int i = 0; i < ninst; i++) {
+			require(mach[i] > 0 && std::isfinite(mach[i]) && std::isfinite(aoa[i]),
+			        "invalid Mach number or angle of attack");
+			fvx_physics_cfg p = d->pc;
+			p.Minf = mach[i];
+			p.aoa = aoa[i];
+			b->gas.push_back(fvx::make_gas(p));
+			b->aoa.push_back(aoa[i]);
+			all[i] = i;
+		}
+		b->d_gas.upload(b->gas);
+		b->d_all.upload(all);
+		const size_t n = (size_t)d->m->nelem;
+		b->d_grad.alloc(8*n*ninst);
+		b->d_rgrad.alloc(8*n*ninst);
+		b->d_scratch.alloc((size_t)fvx::functional_blocks()*5);
+		*out = b.release();
+	});
+}
+
+fvx_status fvx_batch_destroy(fvx_batch b)
+{
+	return guarded([&] { delete b; });
+}
+
+fvx_status fvx_batch_freestream(fvx_batch b, double* out)
+{
+	return guarded([&] {
+		require(b && out, "null argument");
+		for(int i = 0; i < b->ninst; i++)
+			for(int k = 0; k < 4; k++) out[4*i + k] = b->gas[i].uinf[k];
+	});
+}
+
+fvx_status fvx_batch_compute_residual(fvx_batch b, const double* u, double* r, int gettimesteps,
+                                      double* dt, void* stream)
+{
+	return guarded([&] {
+		require(b && u && r, "null argument");
+		require(!gettimesteps || dt, "time steps requested without an output array");
+		fvx::BatchArg ba;
+		ba.list = b->d_all.p;
+		ba.gp = b->d_gas.p;
+		fvx::launch_residual(*b->d, u, r, gettimesteps, dt, static_cast<hipStream_t>(stream), ba,
+		                     b->ninst, b->d_grad.p, b->d_rgrad.p);
+	});
+}
+
+fvx_status fvx_batch_steady_solve(fvx_batch b, const fvx_solver_cfg* cfg, double* u,
+                                  fvx_solve_stats* stats, void* stream)
+{
+	return guarded([&] {
+		require(b && cfg && u && stats, "null argument");
+		std::vector<fvx_solve_stats> v;
+		fvx::steady_core(*b->d, *cfg, u, b->ninst, b->d_gas.p, v,
+		                 static_cast<hipStream_t>(stream));
+		for(int i = 0; i < b->ninst; i++) stats[i] = v[i];
+	});
+}
+
+fvx_status fvx_batch_functionals(fvx_batch b, const double* u, int wall_tag, double* out,
+                                 void* stream)
+{
+	return guarded([&] {
+		require(b && u && out, "null argument");
+		hipStream_t s = static_cast<hipStream_t>(stream);
+		fvx_disc_s& d = *b->d;
+		const size_t n = (size_t)d.m->nelem;
+		const int nb = fvx::functional_blocks();
+		std::vector<double> part((size_t)nb*5);
+		for(int i = 0; i < b->ninst; i++) {
+			fvx::BatchArg ba;
+			ba.list = b->d_all.p + i;
+			ba.gp = b->d_gas.p;
+			fvx::launch_gradients(d, u, b->d_grad.p, nullptr, 1, s, ba, 1);
+			fvx::launch_functionals(d, u + 4*n*i, b->d_grad.p + 8*n*i, wall_tag, b->d_scratch.p, s,
+			                        &b->gas[i], b->aoa[i]);
+			FVX_HIP(hipMemcpyAsync(part.data(), b->d_scratch.p, sizeof(double)*part.size(),
+			                       hipMemcpyDeviceToHost, s));
+			FVX_HIP(hipStreamSynchronize(s));
+			double acc[5] = {0, 0, 0, 0, 0};
+			for(int k = 0; k < nb; k++)
+				for(int j = 0; j < 5; j++) acc[j] += part[(size_t)k*5 + j];
+			require(acc[3] > 0, "no boundary face carries the requested wall marker");
+			out[4*i + 0] = acc[0]/acc[3];
+			out[4*i + 1] = acc[1]/acc[3];
+			out[4*i + 2] = acc[2]/acc[3];
+			out[4*i + 3] = std::sqrt(acc[4]);
+		}
+	});
+}
+
+/* ------------------------------------------------------------------ autotuning -------- */
+
+fvx_status fvx_autotune(fvx_disc d, const double* u, int reps, fvx_tune_row* rows, int cap,
+                        int* nrows, void* stream)
+{
+	return guarded([&] {
+		require(d && u, "null argument");
+		require(reps >= 1, "at least one timed repetition");
+		std::vector<fvx_tune_row> v;
+		fvx::autotune_launch(*d, u, reps, v, static_cast<hipStream_t>(stream));
+		if(nrows) *nrows = (int)v.size();
+		if(rows) for(int i = 0; i < (int)v.size() && i < cap; i++) rows[i] = v[i];
+	});
+}
+
+fvx_status fvx_set_launch(fvx_disc d, int kernel, int block, int xcd)
+{
+	return guarded([&] {
+		require(d != nullptr, "null handle");
+		require(block == 64 || block == 128 || block == 256 || block == 512,
+		        "block must be 64, 128, 256 or 512");
+		require(xcd == 0 || xcd == 1, "xcd must be 0 or 1");
+		if(kernel == FVX_KERNEL_GRADIENTS) { d->tune.grad_block = block; d->tune.grad_xcd = xcd; }
+		else if(kernel == FVX_KERNEL_FLUX) { d->tune.flux_block = block; d->tune.flux_xcd = xcd; }
+		else if(kernel == FVX_KERNEL_JACOBIAN) {
+			require(block <= 256, "the Jacobian kernel takes at most 256 threads per workgroup");
+			d->tune.jac_block = block; d->tune.jac_xcd = xcd;
+		}
+		else require(false, "kernel has no tunable launch shape");
+	});
+}
+
+fvx_status fvx_get_launch(fvx_disc d, int kernel, int* block, int* xcd)
+{
+	return guarded([&] {
+		require(d && block && xcd, "null argument");
+		if(kernel == FVX_KERNEL_GRADIENTS) { *block = d->tune.grad_block; *xcd = d->tune.grad_xcd; }
+		else if(kernel == FVX_KERNEL_FLUX) { *block = d->tune.flux_block; *xcd = d->tune.flux_xcd; }
+		else if(kernel == FVX_KERNEL_JACOBIAN) { *block = d->tune.jac_block; *xcd = d->tune.jac_xcd; }
+		else require(false, "kernel has no tunable launch shape");
+	});
+}
+
+fvx_status fvx_autotune_pc(fvx_disc d, const double* u, double cfl, int restart, double rtol,
+                           int precision, fvx_tune_row* rows, int cap, int* nrows, int* best_pc,
+                           int* best_sweeps, void* stream)
+{
+	return guarded([&] {
+		require(d && u && best_pc && best_sweeps, "null argument");
+		require(cfl > 0 && rtol > 0, "CFL number and tolerance must be positive");
+		std::vector<fvx_tune_row> v;
+		fvx::autotune_pc(*d, u, cfl, restart, rtol, precision, v, *best_pc, *best_sweeps,
+		                 static_cast<hipStream_t>(stream));
+		if(nrows) *nrows = (int)v.size();
+		if(rows) for(int i = 0; i < (int)v.size() && i < cap; i++) rows[i] = v[i];
 	});
 }
 

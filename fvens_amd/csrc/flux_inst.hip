@@ -16,12 +16,17 @@ void FVX_CAT(launch_flux_, FVX_FLUX_ID)(int mode, bool visc, int ncell, const do
                                         const double* rgrad, const double* grad,
                                         const int* nbr, const int* face, const FaceGeom* fg,
                                         const double* centre, const double* area,
-                                        const int* bcid, const GasParams& gp, const BCTable& bt,
-                                        double* res, double* dt, int getdt, hipStream_t s)
+                                        const int* bcid, const GasParams* gpa,
+                                        const BCTable* bta,
+                                        double* res, double* dt, int getdt,
+                                        const BatchArg& ba, int nact, int block, int xcd,
+                                        hipStream_t s)
 {
-	const int nblk = (ncell + CELLS_PER_BLOCK - 1)/CELLS_PER_BLOCK;
-#define FVX_LAUNCH(M, V) hipLaunchKernelGGL((k_flux<FVX_FLUX_ID, M, V>), dim3(nblk), dim3(BLOCK), \
-		0, s, ncell, u, rgrad, grad, nbr, face, fg, centre, area, bcid, gp, bt, res, dt, getdt)
+	const int cpb = block/4;
+	const int nblk = (ncell + cpb - 1)/cpb;
+#define FVX_LAUNCH(M, V) hipLaunchKernelGGL((k_flux<FVX_FLUX_ID, M, V>), dim3(nblk, nact), \
+		dim3(block), 0, s, ncell, u, rgrad, grad, nbr, face, fg, centre, area, bcid, gpa, bta, \
+		ba, res, dt, getdt, xcd)
 	if(!visc) {
 		if(mode == 0) FVX_LAUNCH(0, false);
 		else if(mode == 1) FVX_LAUNCH(1, false);

@@ -94,16 +94,27 @@ __device__ __forceinline__ void load_grad(const double* __restrict__ g, int c, d
 }
 
 template <int FLUX, int MODE, bool VISC>
-__global__ __launch_bounds__(BLOCK)
+__global__ __launch_bounds__(MAX_BLOCK)
 void k_flux(int ncell, const double* __restrict__ u, const double* __restrict__ rgrad,
             const double* __restrict__ grad, const int* __restrict__ nbr,
             const int* __restrict__ face, const FaceGeom* __restrict__ fgeom,
             const double* __restrict__ centre, const double* __restrict__ area,
-            const int* __restrict__ bcid, const GasParams gp, const BCTable bt,
-            double* __restrict__ res, double* __restrict__ dt, const int getdt)
+            const int* __restrict__ bcid, const GasParams* __restrict__ gpa,
+            const BCTable* __restrict__ bta, const BatchArg ba, double* __restrict__ res,
+            double* __restrict__ dt, const int getdt, const int xcd)
 {
+	// instance: state, gradients, residual and time steps are instance-major; gpa holds the
+	// gas parameters of each instance (one entry for a single solve)
+	const long I = ba.list ? ba.list[blockIdx.y] : 0;
+	const GasParams& gp = gpa[I];
+	const BCTable& bt = *bta;
+	u += I*4L*ncell;
+	if(rgrad) rgrad += I*8L*ncell;
+	if(grad) grad += I*8L*ncell;
+	res += I*4L*ncell;
+	if(dt) dt += I*(long)ncell;
 	const int q = threadIdx.x & 3;
-	const int cell = xcd_block()*CELLS_PER_BLOCK + (threadIdx.x >> 2);
+	const int cell = xcd_block(xcd)*(int)(blockDim.x >> 2) + (threadIdx.x >> 2);
 	const bool valid = cell < ncell;
 	const int cc = valid ? cell : 0;
 	const int nb = nbr[cc*4 + q];

@@ -29,6 +29,23 @@ struct alignas(16) FaceGeom {
 	double nx, ny, len, mx, my, pad;
 };
 
+/// Instance addressing of a launch. A batched launch covers gridDim.y instances whose ids
+/// are list[blockIdx.y]; per-instance arrays are instance-major (instance i of an array of
+/// stride S starts at i*S) and each instance has its own gas / boundary parameters. A
+/// single-instance launch has list == nullptr, instance 0 and the by-value parameters.
+struct BatchArg {
+	const int* list = nullptr;
+	const GasParams* gp = nullptr;         ///< per-instance gas parameters (null: the single one)
+	const double* cfl = nullptr;           ///< per-instance CFL numbers (Jacobian kernel)
+};
+
+/// Host-side description of a launch's instances: the device list of active instance ids
+/// (null for a single instance) and how many there are (gridDim.y).
+struct Inst {
+	const int* list = nullptr;
+	int nact = 1;
+};
+
 /// Device buffer owned by a handle.
 template <class T> struct DBuf {
 	T* p = nullptr;
@@ -98,9 +115,19 @@ struct SolverWork;
 void destroy_solver_work(SolverWork* w);
 }
 
+namespace fvx {
+/// Launch shapes of the cell kernels (threads per workgroup, XCD-aware block order), chosen
+/// by the autotuner; results do not depend on them.
+struct Tuning {
+	int flux_block = 256, grad_block = 256, jac_block = 256;
+	int flux_xcd = 1, grad_xcd = 1, jac_xcd = 1;
+};
+}
+
 /// Discretisation handle: configuration plus per-mesh coefficient arrays and scratch.
 struct fvx_disc_s {
 	mutable fvx::Profiler prof;
+	fvx::Tuning tune;
 	fvx_mesh_s* m = nullptr;
 	fvx_physics_cfg pc;
 	fvx_numerics_cfg nc;
@@ -108,6 +135,8 @@ struct fvx_disc_s {
 	fvx::BCTable bct;
 	std::vector<int> bcid;                 // per boundary face: row of bct
 	fvx::DBuf<int> d_bcid;                 // [nbface]
+	fvx::DBuf<fvx::GasParams> d_gas;       // [1] device copy of gas
+	fvx::DBuf<fvx::BCTable> d_bct;         // [1] device copy of bct
 	fvx::DBuf<double> d_gcoef;             // [nelem][4][2] gradient stencil coefficients
 	fvx::DBuf<double> d_eps2;              // [nelem] Venkatakrishnan (K h)^3
 	fvx::DBuf<double> d_grad;              // [nelem][8] unlimited primitive gradients
@@ -117,11 +146,25 @@ struct fvx_disc_s {
 	~fvx_disc_s() { fvx::destroy_solver_work(work); }
 };
 
+/// Operating points solved together on one discretisation: per-instance Mach number and
+/// angle of attack (gas parameters and free stream), everything else shared.
+struct fvx_batch_s {
+	fvx_disc_s* d = nullptr;
+	int ninst = 0;
+	std::vector<double> aoa;
+	std::vector<fvx::GasParams> gas;
+	fvx::DBuf<fvx::GasParams> d_gas;
+	fvx::DBuf<int> d_all;                  // 0 .. ninst-1
+	fvx::DBuf<double> d_grad, d_rgrad;     // [ninst][nelem][8]
+	fvx::DBuf<double> d_scratch;           // functional partial sums
+};
+
 /// Block matrix on the cell graph: vals[cell][slot][4][4], slot 0 the diagonal block and slot
 /// 1+k the block of neighbour nbr[cell][k] (zero where there is none).
 struct fvx_matrix_s {
 	const fvx_mesh_s* m = nullptr;
 	int prec = FVX_PREC_F64;
+	int ninst = 1;                         // instances stored, 80*ncell values apart
 	fvx::DBuf<double> v64;
 	fvx::DBuf<float> v32;
 	double* f64() const { return v64.p; }
@@ -134,7 +177,9 @@ struct fvx_matrix_s {
 struct fvx_pc_s {
 	const fvx_mesh_s* m = nullptr;
 	int type = FVX_PC_NONE, prec = FVX_PREC_F64;
+	int ninst = 1;                         // instances stored, 80*ncell values apart
 	int sweeps = 1;                        // stationary iterations per application
+	fvx::DBuf<double> one;                 // the constant 1.0 (identity preconditioner)
 	const fvx_matrix_s* A = nullptr;       // matrix of the last set-up
 	fvx::DBuf<double> t1, t2;              // residual and correction for sweeps > 1
 	int ncol = 0;
@@ -151,52 +196,63 @@ namespace fvx {
 void build_mesh(fvx_mesh_s& m, const fvx_mesh_desc& d);
 void rcm_order(const fvx_mesh_desc& d, int* perm);
 void build_disc(fvx_disc_s& d);
+GasParams make_gas(const fvx_physics_cfg& p);
 
-// kernel launchers (residual.hip)
+// kernel launchers (spatial.hip, flux_inst.hip); a batched launch passes the instance list
+// and per-instance parameters in BatchArg and the number of instances in nact
 void launch_gradients(const fvx_disc_s& d, const double* u, double* grad, double* rgrad,
-                      int conserved, hipStream_t s);
+                      int conserved, hipStream_t s, const BatchArg& ba = BatchArg(),
+                      int nact = 1);
 void launch_residual(const fvx_disc_s& d, const double* u, double* r, int gettimesteps,
-                     double* dt, hipStream_t s);
+                     double* dt, hipStream_t s, const BatchArg& ba = BatchArg(), int nact = 1,
+                     double* grad = nullptr, double* rgrad = nullptr);
 void launch_functionals(const fvx_disc_s& d, const double* u, const double* cgrad, int walltag,
-                        double* dev_out, hipStream_t s);
+                        double* dev_out, hipStream_t s, const GasParams* gp = nullptr,
+                        double aoa = 0.0);
 int functional_blocks();
 
 // Jacobian assembly (jacobian.hip)
 bool jacobian_supported(const fvx_disc_s& d);
 void launch_jacobian(const fvx_disc_s& d, const double* u, void* vals, int prec,
-                     const double* dt, double cfl, double* dtm, hipStream_t s);
+                     const double* dt, double cfl, double* dtm, hipStream_t s,
+                     const BatchArg& ba = BatchArg(), int nact = 1);
 
-// matrix, preconditioner and vector kernels (linalg.hip)
-void launch_spmv(const fvx_matrix_s& A, const double* x, double* y, hipStream_t s);
+// matrix, preconditioner and vector kernels (linalg.hip); s* arguments are the per-instance
+// strides of the vectors, in doubles
+void launch_spmv(const fvx_matrix_s& A, const double* x, long sx, double* y, long sy,
+                 const Inst& L, hipStream_t s);
 void launch_pseudotime(fvx_matrix_s& A, const double* dt, double cfl, double* dtm,
                        hipStream_t s);
-void launch_pc_setup(fvx_pc_s& p, const fvx_matrix_s& A, hipStream_t s);
-void launch_pc_apply(const fvx_pc_s& p, const double* r, double* z, hipStream_t s);
+void launch_pc_setup(fvx_pc_s& p, const fvx_matrix_s& A, const Inst& L, hipStream_t s);
+void launch_pc_apply(const fvx_pc_s& p, const double* r, long sr, double* z, long sz,
+                     const Inst& L, hipStream_t s);
 int reduction_blocks();
 int max_dots();
-void launch_mdot(long N, const double* V, long ldv, int nv, const double* w, double* partial,
-                 double* out, hipStream_t s);
-void launch_gs_update(long N, const double* V, long ldv, int nv, const double* h, double* w,
-                      double* partial, double* nrm2, hipStream_t s);
-void launch_normalise(long N, double* w, const double* nrm2, hipStream_t s);
-void launch_scale_copy(long N, const double* b, const double* nrm2, double* out, hipStream_t s);
-void launch_lincomb(long N, const double* Z, long ldz, int nv, const double* y, double* x,
-                    hipStream_t s);
+long partial_stride();
+void launch_mdot(long N, const double* V, long ldv, long sv, int nv, const double* w, long sw,
+                 double* partial, double* out, long so, const Inst& L, hipStream_t s);
+void launch_gs_update(long N, const double* V, long ldv, long sv, int nv, const double* h,
+                      long sh, double* w, long sw, double* partial, double* nrm2, long sn,
+                      const Inst& L, hipStream_t s);
+void launch_scale(long N, const double* b, long sb, const double* nrm2, long sn, double* out,
+                  long so, const Inst& L, hipStream_t s);
+void launch_lincomb(long N, const double* Z, long ldz, long sz, int nv, const double* y,
+                    long sy, double* x, long sx, const Inst& L, hipStream_t s);
 void launch_resnorm(int n, const double* r, const double* area, double* partial, double* out,
-                    hipStream_t s);
-void launch_mf_prep(long N, const double* u, const double* x, const double* xx, double eps,
-                    double* aux, hipStream_t s);
-void launch_mf_finish(int n, const double* x, const double* dtm, const double* ru,
-                      const double* raux, const double* xx, double eps, double* y,
-                      hipStream_t s);
+                    long so, const Inst& L, hipStream_t s);
+void launch_mf_prep(long N, const double* u, const double* x, long sx, const double* xx,
+                    long sn, double eps, double* aux, const Inst& L, hipStream_t s);
+void launch_mf_finish(int n, const double* x, long sx, const double* dtm, const double* ru,
+                      const double* raux, const double* xx, long sn, double eps, double* y,
+                      long sy, const Inst& L, hipStream_t s);
 void launch_update_implicit(int n, double* u, const double* du, int full, double min_relax,
-                            double gamma, hipStream_t s);
+                            double gamma, const Inst& L, hipStream_t s);
 void launch_update_explicit(int n, double* u, const double* r, const double* dt,
-                            const double* area, double cfl, hipStream_t s);
+                            const double* area, double cfl, const Inst& L, hipStream_t s);
 
 // host side of the implicit solver (solver.cpp)
-void matrix_init(fvx_matrix_s& A, const fvx_mesh_s* m, int prec);
-void pc_init(fvx_pc_s& p, const fvx_mesh_s* m, int type, int prec);
+void matrix_init(fvx_matrix_s& A, const fvx_mesh_s* m, int prec, int ninst = 1);
+void pc_init(fvx_pc_s& p, const fvx_mesh_s* m, int type, int prec, int ninst = 1);
 void pc_set_sweeps(fvx_pc_s& p, int sweeps);
 void matrix_export_bsr(const fvx_matrix_s& A, int* rowptr, int* colind, double* vals);
 int matrix_bsr_nnzb(const fvx_mesh_s& m);
@@ -207,6 +263,15 @@ int linear_solve(const fvx_matrix_s& A, const fvx_pc_s& p, const double* b, doub
                  int restart, double rtol, hipStream_t s);
 void steady_solve(fvx_disc_s& d, const fvx_solver_cfg& cfg, double* u, fvx_solve_stats& st,
                   hipStream_t s);
+void steady_core(fvx_disc_s& d, const fvx_solver_cfg& c, double* u, int ninst,
+                 const GasParams* gp, std::vector<fvx_solve_stats>& stats, hipStream_t s);
+
+// autotuner (tune.cpp)
+void autotune_launch(fvx_disc_s& d, const double* u, int reps, std::vector<fvx_tune_row>& rows,
+                     hipStream_t s);
+void autotune_pc(fvx_disc_s& d, const double* u, double cfl, int restart, double rtol,
+                 int precision, std::vector<fvx_tune_row>& rows, int& best_pc, int& best_sweeps,
+                 hipStream_t s);
 }
 
 #endif

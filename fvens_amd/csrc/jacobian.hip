@@ -238,12 +238,22 @@ __global__ __launch_bounds__(BLOCK)
 void k_jacobian(int ncell, const double* __restrict__ u, const int* __restrict__ nbr,
                 const int* __restrict__ face, const FaceGeom* __restrict__ fgeom,
                 const double* __restrict__ centre, const int* __restrict__ bcid,
-                const GasParams gp, const BCTable bt, VT* __restrict__ vals,
-                const double* __restrict__ dt, const double* __restrict__ area, double cfl,
-                double* __restrict__ dtm)
+                const GasParams* __restrict__ gpa, const BCTable* __restrict__ bta,
+                const BatchArg ba, VT* __restrict__ vals, const double* __restrict__ dt,
+                const double* __restrict__ area, double cfl0, double* __restrict__ dtm,
+                const int xcd)
 {
+	// instance: state, matrix, time steps and CFL number are per instance; gpa holds the gas
+	// parameters of each instance (one entry for a single solve)
+	const long I = ba.list ? ba.list[blockIdx.y] : 0;
+	const GasParams& gp = gpa[I];
+	const BCTable& bt = *bta;
+	const double cfl = ba.cfl ? ba.cfl[I] : cfl0;
+	u += I*4L*ncell;
+	vals += I*80L*ncell;
+	if(dt) { dt += I*(long)ncell; dtm += I*(long)ncell; }
 	const int q = threadIdx.x & 3;
-	const int cell = xcd_block()*CELLS_PER_BLOCK + (threadIdx.x >> 2);
+	const int cell = xcd_block(xcd)*(int)(blockDim.x >> 2) + (threadIdx.x >> 2);
 	const bool valid = cell < ncell;
 	const int cc = valid ? cell : 0;
 	const int nb = nbr[cc*4 + q];
@@ -299,28 +309,33 @@ void k_jacobian(int ncell, const double* __restrict__ u, const int* __restrict__
 
 template <int FLUX, bool VISC>
 static void launch_jac_t(const fvx_disc_s& d, const double* u, void* vals, int prec,
-                         const double* dt, double cfl, double* dtm, hipStream_t s)
+                         const double* dt, double cfl, double* dtm, hipStream_t s,
+                         const BatchArg& ba, int nact)
 {
 	const fvx_mesh_s* m = d.m;
 	const int n = m->nelem;
-	const int nblk = (n + CELLS_PER_BLOCK - 1)/CELLS_PER_BLOCK;
+	const int block = d.tune.jac_block, cpb = block/4;
+	const dim3 g((n + cpb - 1)/cpb, nact);
 	if(prec == FVX_PREC_F32)
-		hipLaunchKernelGGL((k_jacobian<FLUX, VISC, float>), dim3(nblk), dim3(BLOCK), 0, s, n, u,
+		hipLaunchKernelGGL((k_jacobian<FLUX, VISC, float>), g, dim3(block), 0, s, n, u,
 		                   m->d_nbr.p, m->d_face.p, m->d_fgeom.p, m->d_centre.p, d.d_bcid.p,
-		                   d.gas, d.bct, static_cast<float*>(vals), dt, m->d_area.p, cfl, dtm);
+		                   ba.gp ? ba.gp : d.d_gas.p, d.d_bct.p, ba, static_cast<float*>(vals), dt,
+		                   m->d_area.p, cfl, dtm, d.tune.jac_xcd);
 	else
-		hipLaunchKernelGGL((k_jacobian<FLUX, VISC, double>), dim3(nblk), dim3(BLOCK), 0, s, n, u,
+		hipLaunchKernelGGL((k_jacobian<FLUX, VISC, double>), g, dim3(block), 0, s, n, u,
 		                   m->d_nbr.p, m->d_face.p, m->d_fgeom.p, m->d_centre.p, d.d_bcid.p,
-		                   d.gas, d.bct, static_cast<double*>(vals), dt, m->d_area.p, cfl, dtm);
+		                   ba.gp ? ba.gp : d.d_gas.p, d.d_bct.p, ba, static_cast<double*>(vals),
+		                   dt, m->d_area.p, cfl, dtm, d.tune.jac_xcd);
 	FVX_HIP(hipGetLastError());
 }
 
 template <int FLUX>
 static void launch_jac_v(const fvx_disc_s& d, const double* u, void* vals, int prec,
-                         const double* dt, double cfl, double* dtm, hipStream_t s)
+                         const double* dt, double cfl, double* dtm, hipStream_t s,
+                         const BatchArg& ba, int nact)
 {
-	if(d.pc.viscous) launch_jac_t<FLUX, true>(d, u, vals, prec, dt, cfl, dtm, s);
-	else launch_jac_t<FLUX, false>(d, u, vals, prec, dt, cfl, dtm, s);
+	if(d.pc.viscous) launch_jac_t<FLUX, true>(d, u, vals, prec, dt, cfl, dtm, s, ba, nact);
+	else launch_jac_t<FLUX, false>(d, u, vals, prec, dt, cfl, dtm, s, ba, nact);
 }
 
 bool jacobian_supported(const fvx_disc_s& d)
@@ -337,15 +352,16 @@ bool jacobian_supported(const fvx_disc_s& d)
 }
 
 void launch_jacobian(const fvx_disc_s& d, const double* u, void* vals, int prec,
-                     const double* dt, double cfl, double* dtm, hipStream_t s)
+                     const double* dt, double cfl, double* dtm, hipStream_t s,
+                     const BatchArg& ba, int nact)
 {
 	d.prof.start(PROF_JACOBIAN, s);
 	switch(d.nc.conv_numflux_jac) {
-	case FVX_FLUX_ROE:  launch_jac_v<0>(d, u, vals, prec, dt, cfl, dtm, s); break;
-	case FVX_FLUX_HLLC: launch_jac_v<1>(d, u, vals, prec, dt, cfl, dtm, s); break;
-	case FVX_FLUX_HLL:  launch_jac_v<2>(d, u, vals, prec, dt, cfl, dtm, s); break;
-	case FVX_FLUX_LLF:  launch_jac_v<3>(d, u, vals, prec, dt, cfl, dtm, s); break;
-	case FVX_FLUX_AUSM: launch_jac_v<4>(d, u, vals, prec, dt, cfl, dtm, s); break;
+	case FVX_FLUX_ROE:  launch_jac_v<0>(d, u, vals, prec, dt, cfl, dtm, s, ba, nact); break;
+	case FVX_FLUX_HLLC: launch_jac_v<1>(d, u, vals, prec, dt, cfl, dtm, s, ba, nact); break;
+	case FVX_FLUX_HLL:  launch_jac_v<2>(d, u, vals, prec, dt, cfl, dtm, s, ba, nact); break;
+	case FVX_FLUX_LLF:  launch_jac_v<3>(d, u, vals, prec, dt, cfl, dtm, s, ba, nact); break;
+	case FVX_FLUX_AUSM: launch_jac_v<4>(d, u, vals, prec, dt, cfl, dtm, s, ba, nact); break;
 	default: throw Error(FVX_ERR_UNSUPPORTED, "flux Jacobian not available for this flux");
 	}
 	d.prof.stop(s);

@@ -110,40 +110,25 @@ __device__ __forceinline__ int4 load_nbr(const int* nbr, int c)
 	return reinterpret_cast<const int4*>(nbr)[c];
 }
 
-/* ------------------------------------------------------------------ matrix ------------- */
-
-/// y = A x; one quad per cell, lane q computes row q.
-template <class VT>
-__global__ __launch_bounds__(BLOCK)
-void k_spmv(int n, const VT* __restrict__ vals, const int* __restrict__ nbr,
-            const double* __restrict__ x, double* __restrict__ y)
+/* The kernels below take an instance list `ilist` (null: one instance, index 0) and run
+ * gridDim.y instances; every per-instance array argument comes with its instance stride. */
+__device__ __forceinline__ long inst_of(const int* ilist)
 {
-	const int q = threadIdx.x & 3;
-	const int c = xcd_block()*CELLS_PER_BLOCK + (threadIdx.x >> 2);
-	if(c >= n) return;
-	const int4 nb = load_nbr(nbr, c);
-	const int nbs[4] = { nb.x, nb.y, nb.z, nb.w };
-	const VT* blk = vals + (size_t)c*80 + 4*q;
-	double a[4], xv[4];
-	load_row<VT>(blk, a);
-	load4(x + 4*c, xv);
-	double acc = dot4(a, xv);
-	for(int k = 0; k < 4; k++) {
-		if(nbs[k] < 0) continue;
-		load_row<VT>(blk + 16*(1 + k), a);
-		load4(x + 4*nbs[k], xv);
-		acc += dot4(a, xv);
-	}
-	y[4*c + q] = acc;
+	return ilist ? ilist[blockIdx.y] : 0;
 }
 
-/// y = b - A x
-template <class VT>
+/* ------------------------------------------------------------------ matrix ------------- */
+
+/// y = A x (or y = b - A x with RES); one quad per cell, lane q computes row q.
+template <class VT, bool RES>
 __global__ __launch_bounds__(BLOCK)
-void k_spmv_residual(int n, const VT* __restrict__ vals, const int* __restrict__ nbr,
-                     const double* __restrict__ x, const double* __restrict__ b,
-                     double* __restrict__ y)
+void k_spmv(int n, const int* __restrict__ ilist, const VT* __restrict__ vals, long sa,
+            const int* __restrict__ nbr, const double* __restrict__ x, long sx,
+            const double* __restrict__ b, long sb, double* __restrict__ y, long sy)
 {
+	const long I = inst_of(ilist);
+	vals += I*sa; x += I*sx; y += I*sy;
+	if constexpr(RES) b += I*sb;
 	const int q = threadIdx.x & 3;
 	const int c = xcd_block()*CELLS_PER_BLOCK + (threadIdx.x >> 2);
 	if(c >= n) return;
@@ -160,13 +145,17 @@ void k_spmv_residual(int n, const VT* __restrict__ vals, const int* __restrict__
 		load4(x + 4*nbs[k], xv);
 		acc += dot4(a, xv);
 	}
-	y[4*c + q] = b[4*c + q] - acc;
+	if constexpr(RES) y[4*c + q] = b[4*c + q] - acc;
+	else y[4*c + q] = acc;
 }
 
 /// y += x
 __global__ __launch_bounds__(BLOCK)
-void k_axpy1(long N, const double* __restrict__ x, double* __restrict__ y)
+void k_axpy1(long N, const int* __restrict__ ilist, const double* __restrict__ x, long sx,
+             double* __restrict__ y, long sy)
 {
+	const long I = inst_of(ilist);
+	x += I*sx; y += I*sy;
 	const long stride = (long)gridDim.x*blockDim.x;
 	for(long e = (long)blockIdx.x*blockDim.x + threadIdx.x; e < N; e += stride) y[e] += x[e];
 }
@@ -192,8 +181,11 @@ void k_pseudotime(int n, VT* __restrict__ vals, const double* __restrict__ dt,
 /// off-diagonal blocks (symmetric Gauss-Seidel sweeps read them from the factor array).
 template <class VTA, class VTF>
 __global__ __launch_bounds__(BLOCK)
-void k_invert_diag(int n, const VTA* __restrict__ vals, VTF* __restrict__ fac, int copy_off)
+void k_invert_diag(int n, const int* __restrict__ ilist, const VTA* __restrict__ vals,
+                   VTF* __restrict__ fac, long sa, int copy_off)
 {
+	const long I = inst_of(ilist);
+	vals += I*sa; fac += I*sa;
 	const int c = blockIdx.x*blockDim.x + threadIdx.x;
 	if(c >= n) return;
 	double a[16], inv[16];
@@ -210,9 +202,11 @@ void k_invert_diag(int n, const VTA* __restrict__ vals, VTF* __restrict__ fac, i
 /// z = D^-1 r, one quad per cell.
 template <class VTF>
 __global__ __launch_bounds__(BLOCK)
-void k_jacobi_apply(int n, const VTF* __restrict__ fac, const double* __restrict__ r,
-                    double* __restrict__ z)
+void k_jacobi_apply(int n, const int* __restrict__ ilist, const VTF* __restrict__ fac, long sa,
+                    const double* __restrict__ r, long sr, double* __restrict__ z, long sz)
 {
+	const long I = inst_of(ilist);
+	fac += I*sa; r += I*sr; z += I*sz;
 	const int q = threadIdx.x & 3;
 	const int c = xcd_block()*CELLS_PER_BLOCK + (threadIdx.x >> 2);
 	if(c >= n) return;
@@ -227,10 +221,13 @@ void k_jacobi_apply(int n, const VTF* __restrict__ fac, const double* __restrict
 /// colours only (the first forward pass from z = 0, where the other colours hold no value yet).
 template <class VTF, bool ALL>
 __global__ __launch_bounds__(BLOCK)
-void k_sgs_colour(int cnt, const int* __restrict__ list, const VTF* __restrict__ fac,
-                  const int* __restrict__ nbr, const unsigned char* __restrict__ lowmask,
-                  const double* __restrict__ r, double* __restrict__ z)
+void k_sgs_colour(int cnt, const int* __restrict__ list, const int* __restrict__ ilist,
+                  const VTF* __restrict__ fac, long sa, const int* __restrict__ nbr,
+                  const unsigned char* __restrict__ lowmask, const double* __restrict__ r,
+                  long sr, double* __restrict__ z, long sz)
 {
+	const long I = inst_of(ilist);
+	fac += I*sa; r += I*sr; z += I*sz;
 	const int q = threadIdx.x & 3;
 	const int idx = xcd_block()*CELLS_PER_BLOCK + (threadIdx.x >> 2);
 	const bool valid = idx < cnt;
@@ -256,10 +253,13 @@ void k_sgs_colour(int cnt, const int* __restrict__ list, const VTF* __restrict__
 /// Forward substitution of block ILU(0) for one colour: y_i = r_i - sum_{lower k} L_ik y_k.
 template <class VTF>
 __global__ __launch_bounds__(BLOCK)
-void k_ilu_fwd(int cnt, const int* __restrict__ list, const VTF* __restrict__ fac,
-               const int* __restrict__ nbr, const unsigned char* __restrict__ lowmask,
-               const double* __restrict__ r, double* __restrict__ z)
+void k_ilu_fwd(int cnt, const int* __restrict__ list, const int* __restrict__ ilist,
+               const VTF* __restrict__ fac, long sa, const int* __restrict__ nbr,
+               const unsigned char* __restrict__ lowmask, const double* __restrict__ r, long sr,
+               double* __restrict__ z, long sz)
 {
+	const long I = inst_of(ilist);
+	fac += I*sa; r += I*sr; z += I*sz;
 	const int q = threadIdx.x & 3;
 	const int idx = xcd_block()*CELLS_PER_BLOCK + (threadIdx.x >> 2);
 	if(idx >= cnt) return;
@@ -283,10 +283,12 @@ void k_ilu_fwd(int cnt, const int* __restrict__ list, const VTF* __restrict__ fa
 /// z_i = D_i^-1 (y_i - sum_{upper j} U_ij z_j).
 template <class VTF>
 __global__ __launch_bounds__(BLOCK)
-void k_ilu_bwd(int cnt, const int* __restrict__ list, const VTF* __restrict__ fac,
-               const int* __restrict__ nbr, const unsigned char* __restrict__ lowmask,
-               double* __restrict__ z)
+void k_ilu_bwd(int cnt, const int* __restrict__ list, const int* __restrict__ ilist,
+               const VTF* __restrict__ fac, long sa, const int* __restrict__ nbr,
+               const unsigned char* __restrict__ lowmask, double* __restrict__ z, long sz)
 {
+	const long I = inst_of(ilist);
+	fac += I*sa; z += I*sz;
 	const int q = threadIdx.x & 3;
 	const int idx = xcd_block()*CELLS_PER_BLOCK + (threadIdx.x >> 2);
 	const bool valid = idx < cnt;
@@ -316,11 +318,13 @@ void k_ilu_bwd(int cnt, const int* __restrict__ list, const VTF* __restrict__ fa
 /// mutually adjacent). Finally slot 0 receives inv(A_ii).
 template <class VTA, class VTF>
 __global__ __launch_bounds__(BLOCK)
-void k_ilu_factor(int cnt, const int* __restrict__ list, const VTA* __restrict__ vals,
-                  VTF* __restrict__ fac, const int* __restrict__ nbr,
-                  const unsigned char* __restrict__ lowmask, const int* __restrict__ colour,
-                  int n)
+void k_ilu_factor(int cnt, const int* __restrict__ list, const int* __restrict__ ilist,
+                  const VTA* __restrict__ vals, VTF* __restrict__ fac, long sa,
+                  const int* __restrict__ nbr, const unsigned char* __restrict__ lowmask,
+                  const int* __restrict__ colour, int n)
 {
+	const long I = inst_of(ilist);
+	vals += I*sa; fac += I*sa;
 	const int idx = blockIdx.x*blockDim.x + threadIdx.x;
 	if(idx >= cnt) return;
 	const int i = list[idx];
@@ -393,8 +397,9 @@ void k_ilu_factor(int cnt, const int* __restrict__ list, const VTA* __restrict__
 
 constexpr int RED_BLOCKS = 512;
 constexpr int MAX_DOTS = 32;
+constexpr long PART = (long)RED_BLOCKS*MAX_DOTS;   // per-instance partial-sum stride
 
-/// Sum over the 256 threads of a block; the result is valid in thread 0.
+/// Sum over the 256 threads of a block; the result is valid in every thread.
 __device__ __forceinline__ double block_sum(double v, double* sh)
 {
 	for(int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -407,9 +412,11 @@ __device__ __forceinline__ double block_sum(double v, double* sh)
 
 /// partial[b][i] = block b's part of dot(V_i, w), i < nv (V_i = V + i*ldv).
 __global__ __launch_bounds__(BLOCK)
-void k_mdot(long N, const double* __restrict__ V, long ldv, int nv,
-            const double* __restrict__ w, double* __restrict__ partial)
+void k_mdot(long N, const int* __restrict__ ilist, const double* __restrict__ V, long ldv,
+            long sv, int nv, const double* __restrict__ w, long sw, double* __restrict__ partial)
 {
+	const long I = inst_of(ilist);
+	V += I*sv; w += I*sw; partial += I*PART;
 	__shared__ double sh[4];
 	double acc[MAX_DOTS];
 	for(int i = 0; i < MAX_DOTS; i++) acc[i] = 0.0;
@@ -426,10 +433,13 @@ void k_mdot(long N, const double* __restrict__ V, long ldv, int nv,
 	}
 }
 
-/// out[i] = sum over blocks of partial[b][i], i < nv (one block, fixed order).
+/// out[i] = sum over blocks of partial[b][i], i < nv (one block per instance, fixed order).
 __global__ __launch_bounds__(BLOCK)
-void k_reduce(int nblk, const double* __restrict__ partial, int nv, double* __restrict__ out)
+void k_reduce(int nblk, const int* __restrict__ ilist, const double* __restrict__ partial,
+              int nv, double* __restrict__ out, long so)
 {
+	const long I = inst_of(ilist);
+	partial += I*PART; out += I*so;
 	__shared__ double sh[4];
 	for(int i = 0; i < nv; i++) {
 		double v = 0.0;
@@ -442,10 +452,12 @@ void k_reduce(int nblk, const double* __restrict__ partial, int nv, double* __re
 
 /// w -= sum_i h_i V_i (classical Gram-Schmidt update) and partial[b][0] = block part of |w|^2.
 __global__ __launch_bounds__(BLOCK)
-void k_gs_update(long N, const double* __restrict__ V, long ldv, int nv,
-                 const double* __restrict__ h, double* __restrict__ w,
-                 double* __restrict__ partial)
+void k_gs_update(long N, const int* __restrict__ ilist, const double* __restrict__ V, long ldv,
+                 long sv, int nv, const double* __restrict__ h, long sh_, double* __restrict__ w,
+                 long sw, double* __restrict__ partial)
 {
+	const long I = inst_of(ilist);
+	V += I*sv; h += I*sh_; w += I*sw; partial += I*PART;
 	__shared__ double sh[4];
 	__shared__ double hs[MAX_DOTS];
 	if(threadIdx.x < MAX_DOTS) hs[threadIdx.x] = threadIdx.x < nv ? h[threadIdx.x] : 0.0;
@@ -463,31 +475,26 @@ void k_gs_update(long N, const double* __restrict__ V, long ldv, int nv,
 	if(threadIdx.x == 0) partial[(size_t)blockIdx.x*MAX_DOTS] = s;
 }
 
-/// w /= sqrt(*nrm2)
+/// out = b / sqrt(*nrm2) (out may alias b)
 __global__ __launch_bounds__(BLOCK)
-void k_normalise(long N, double* __restrict__ w, const double* __restrict__ nrm2)
+void k_scale(long N, const int* __restrict__ ilist, const double* b, long sb,
+             const double* __restrict__ nrm2, long sn, double* out, long so)
 {
+	const long I = inst_of(ilist);
+	b += I*sb; nrm2 += I*sn; out += I*so;
 	const double h = sqrt(nrm2[0]);
 	const long stride = (long)gridDim.x*blockDim.x;
-	for(long e = (long)blockIdx.x*blockDim.x + threadIdx.x; e < N; e += stride) w[e] /= h;
-}
-
-/// out = b / beta, beta = sqrt(*nrm2)
-__global__ __launch_bounds__(BLOCK)
-void k_scale_copy(long N, const double* __restrict__ b, const double* __restrict__ nrm2,
-                  double* __restrict__ out)
-{
-	const double h = sqrt(nrm2[0]);
-	const long stride = (long)gridDim.x*blockDim.x;
-	for(long e = (long)blockIdx.x*blockDim.x + threadIdx.x; e < N; e += stride)
-		out[e] = b[e]/h;
+	for(long e = (long)blockIdx.x*blockDim.x + threadIdx.x; e < N; e += stride) out[e] = b[e]/h;
 }
 
 /// x = sum_k y_k Z_k, k < nv
 __global__ __launch_bounds__(BLOCK)
-void k_lincomb(long N, const double* __restrict__ Z, long ldz, int nv,
-               const double* __restrict__ y, double* __restrict__ x)
+void k_lincomb(long N, const int* __restrict__ ilist, const double* __restrict__ Z, long ldz,
+               long sz, int nv, const double* __restrict__ y, long sy, double* __restrict__ x,
+               long sx)
 {
+	const long I = inst_of(ilist);
+	Z += I*sz; y += I*sy; x += I*sx;
 	__shared__ double ys[MAX_DOTS];
 	if(threadIdx.x < MAX_DOTS) ys[threadIdx.x] = threadIdx.x < nv ? y[threadIdx.x] : 0.0;
 	__syncthreads();
@@ -503,9 +510,11 @@ void k_lincomb(long N, const double* __restrict__ Z, long ldz, int nv,
 /// partial[b][0] = block part of sum_c r[c][3]^2 area[c] (the reference's energy-residual
 /// norm, src/ode/aodesolver.cpp:516-530).
 __global__ __launch_bounds__(BLOCK)
-void k_resnorm(int n, const double* __restrict__ r, const double* __restrict__ area,
-               double* __restrict__ partial)
+void k_resnorm(int n, const int* __restrict__ ilist, const double* __restrict__ r, long sr,
+               const double* __restrict__ area, double* __restrict__ partial)
 {
+	const long I = inst_of(ilist);
+	r += I*sr; partial += I*PART;
 	__shared__ double sh[4];
 	double acc = 0.0;
 	for(int c = blockIdx.x*blockDim.x + threadIdx.x; c < n; c += gridDim.x*blockDim.x) {
@@ -520,9 +529,12 @@ void k_resnorm(int n, const double* __restrict__ r, const double* __restrict__ a
 
 /// aux = u + (eps/|x|) x with |x|^2 in *xx
 __global__ __launch_bounds__(BLOCK)
-void k_mf_prep(long N, const double* __restrict__ u, const double* __restrict__ x,
-               const double* __restrict__ xx, double eps, double* __restrict__ aux)
+void k_mf_prep(long N, const int* __restrict__ ilist, const double* __restrict__ u, long su,
+               const double* __restrict__ x, long sx, const double* __restrict__ xx, long sn,
+               double eps, double* __restrict__ aux, long sa)
 {
+	const long I = inst_of(ilist);
+	u += I*su; x += I*sx; xx += I*sn; aux += I*sa;
 	const double pert = eps/sqrt(xx[0]);
 	const long stride = (long)gridDim.x*blockDim.x;
 	for(long e = (long)blockIdx.x*blockDim.x + threadIdx.x; e < N; e += stride)
@@ -531,10 +543,13 @@ void k_mf_prep(long N, const double* __restrict__ u, const double* __restrict__ 
 
 /// y = dtm x + (r_u - r_aux)/pert, where r_u and r_aux hold -r() at u and at u + pert x.
 __global__ __launch_bounds__(BLOCK)
-void k_mf_finish(int n, const double* __restrict__ x, const double* __restrict__ dtm,
-                 const double* __restrict__ ru, const double* __restrict__ raux,
-                 const double* __restrict__ xx, double eps, double* __restrict__ y)
+void k_mf_finish(int n, const int* __restrict__ ilist, const double* __restrict__ x, long sx,
+                 const double* __restrict__ dtm, const double* __restrict__ ru,
+                 const double* __restrict__ raux, const double* __restrict__ xx, long sn,
+                 double eps, double* __restrict__ y, long sy)
 {
+	const long I = inst_of(ilist);
+	x += I*sx; dtm += I*(long)n; ru += I*4L*n; raux += I*4L*n; xx += I*sn; y += I*sy;
 	const double pert = eps/sqrt(xx[0]);
 	for(int e = blockIdx.x*blockDim.x + threadIdx.x; e < 4*n; e += gridDim.x*blockDim.x)
 		y[e] = dtm[e >> 2]*x[e] + (-raux[e] + ru[e])/pert;
@@ -546,9 +561,11 @@ void k_mf_finish(int n, const double* __restrict__ x, const double* __restrict__
 /// sum runs over conserved components 2 and 3 (y-momentum and energy), not over the two
 /// momenta; that sum is kept as the reference has it.
 __global__ __launch_bounds__(BLOCK)
-void k_update_implicit(int n, double* __restrict__ u, const double* __restrict__ du,
-                       int full, double min_relax, double gamma)
+void k_update_implicit(int n, const int* __restrict__ ilist, double* __restrict__ u,
+                       const double* __restrict__ du, int full, double min_relax, double gamma)
 {
+	const long I = inst_of(ilist);
+	u += I*4L*n; du += I*4L*n;
 	const int c = blockIdx.x*blockDim.x + threadIdx.x;
 	if(c >= n) return;
 	double uc[4], d[4];
@@ -572,10 +589,12 @@ void k_update_implicit(int n, double* __restrict__ u, const double* __restrict__
 
 /// u += cfl dt/area r (SteadyForwardEulerSolver::solve, src/ode/aodesolver.cpp:200-212).
 __global__ __launch_bounds__(BLOCK)
-void k_update_explicit(int n, double* __restrict__ u, const double* __restrict__ r,
-                       const double* __restrict__ dt, const double* __restrict__ area,
-                       double cfl)
+void k_update_explicit(int n, const int* __restrict__ ilist, double* __restrict__ u,
+                       const double* __restrict__ r, const double* __restrict__ dt,
+                       const double* __restrict__ area, double cfl)
 {
+	const long I = inst_of(ilist);
+	u += I*4L*n; r += I*4L*n; dt += I*(long)n;
 	const int c = blockIdx.x*blockDim.x + threadIdx.x;
 	if(c >= n) return;
 	const double f = cfl*dt[c]*1.0/area[c];
@@ -591,16 +610,19 @@ static inline int stream_blocks(long N)
 	const long b = (N + BLOCK - 1)/BLOCK;
 	return (int)(b < 4096 ? (b > 0 ? b : 1) : 4096);
 }
+static inline long mstride(const fvx_mesh_s* m) { return 80L*m->nelem; }
 
-void launch_spmv(const fvx_matrix_s& A, const double* x, double* y, hipStream_t s)
+void launch_spmv(const fvx_matrix_s& A, const double* x, long sx, double* y, long sy,
+                 const Inst& L, hipStream_t s)
 {
 	const int n = A.m->nelem;
+	const dim3 g(quad_blocks(n), L.nact);
 	if(A.prec == FVX_PREC_F32)
-		hipLaunchKernelGGL(k_spmv<float>, dim3(quad_blocks(n)), dim3(BLOCK), 0, s, n,
-		                   A.f32(), A.m->d_nbr.p, x, y);
+		hipLaunchKernelGGL((k_spmv<float, false>), g, dim3(BLOCK), 0, s, n, L.list, A.f32(),
+		                   mstride(A.m), A.m->d_nbr.p, x, sx, nullptr, 0L, y, sy);
 	else
-		hipLaunchKernelGGL(k_spmv<double>, dim3(quad_blocks(n)), dim3(BLOCK), 0, s, n,
-		                   A.f64(), A.m->d_nbr.p, x, y);
+		hipLaunchKernelGGL((k_spmv<double, false>), g, dim3(BLOCK), 0, s, n, L.list, A.f64(),
+		                   mstride(A.m), A.m->d_nbr.p, x, sx, nullptr, 0L, y, sy);
 	FVX_HIP(hipGetLastError());
 }
 
@@ -618,91 +640,98 @@ void launch_pseudotime(fvx_matrix_s& A, const double* dt, double cfl, double* dt
 }
 
 template <class VTA, class VTF>
-static void pc_setup_t(fvx_pc_s& p, const VTA* vals, VTF* fac, hipStream_t s)
+static void pc_setup_t(fvx_pc_s& p, const VTA* vals, VTF* fac, const Inst& L, hipStream_t s)
 {
 	const int n = p.m->nelem;
+	const long sa = mstride(p.m);
 	if(p.type == FVX_PC_JACOBI || p.type == FVX_PC_SGS) {
-		hipLaunchKernelGGL((k_invert_diag<VTA, VTF>), dim3(thread_blocks(n)), dim3(BLOCK), 0, s,
-		                   n, vals, fac, p.type == FVX_PC_SGS ? 1 : 0);
+		hipLaunchKernelGGL((k_invert_diag<VTA, VTF>), dim3(thread_blocks(n), L.nact), dim3(BLOCK),
+		                   0, s, n, L.list, vals, fac, sa, p.type == FVX_PC_SGS ? 1 : 0);
 		FVX_HIP(hipGetLastError());
 	} else if(p.type == FVX_PC_ILU0) {
 		for(int c = 0; c < p.ncol; c++) {
 			const int cnt = p.coff[c+1] - p.coff[c];
 			if(cnt == 0) continue;
-			hipLaunchKernelGGL((k_ilu_factor<VTA, VTF>), dim3(thread_blocks(cnt)), dim3(BLOCK), 0,
-			                   s, cnt, p.d_order.p + p.coff[c], vals, fac, p.m->d_nbr.p,
-			                   p.d_lowmask.p, p.d_colour.p, n);
+			hipLaunchKernelGGL((k_ilu_factor<VTA, VTF>), dim3(thread_blocks(cnt), L.nact),
+			                   dim3(BLOCK), 0, s, cnt, p.d_order.p + p.coff[c], L.list, vals, fac,
+			                   sa, p.m->d_nbr.p, p.d_lowmask.p, p.d_colour.p, n);
 			FVX_HIP(hipGetLastError());
 		}
 	}
 }
 
-void launch_pc_setup(fvx_pc_s& p, const fvx_matrix_s& A, hipStream_t s)
+void launch_pc_setup(fvx_pc_s& p, const fvx_matrix_s& A, const Inst& L, hipStream_t s)
 {
 	p.A = &A;
 	if(p.type == FVX_PC_NONE) return;
+	if(A.ninst < p.ninst) throw Error(FVX_ERR_INVALID, "matrix holds fewer instances than the preconditioner");
 	if(A.prec == FVX_PREC_F32) {
-		if(p.prec == FVX_PREC_F32) pc_setup_t(p, A.f32(), p.f32(), s);
-		else pc_setup_t(p, A.f32(), p.f64(), s);
+		if(p.prec == FVX_PREC_F32) pc_setup_t(p, A.f32(), p.f32(), L, s);
+		else pc_setup_t(p, A.f32(), p.f64(), L, s);
 	} else {
-		if(p.prec == FVX_PREC_F32) pc_setup_t(p, A.f64(), p.f32(), s);
-		else pc_setup_t(p, A.f64(), p.f64(), s);
+		if(p.prec == FVX_PREC_F32) pc_setup_t(p, A.f64(), p.f32(), L, s);
+		else pc_setup_t(p, A.f64(), p.f64(), L, s);
 	}
 }
 
 /// One forward and one backward pass of symmetric Gauss-Seidel over the colours; the first
 /// forward pass (from z = 0) reads lower colours only.
 template <class VTF>
-static void sgs_passes(const fvx_pc_s& p, const VTF* fac, const double* r, double* z,
-                       bool first, hipStream_t s)
+static void sgs_passes(const fvx_pc_s& p, const VTF* fac, const double* r, long sr, double* z,
+                       long sz, bool first, const Inst& L, hipStream_t s)
 {
 	const int* nbr = p.m->d_nbr.p;
+	const long sa = mstride(p.m);
 	for(int c = 0; c < p.ncol; c++) {
 		const int cnt = p.coff[c+1] - p.coff[c];
 		if(cnt == 0) continue;
 		const int* list = p.d_order.p + p.coff[c];
+		const dim3 g(quad_blocks(cnt), L.nact);
 		if(first)
-			hipLaunchKernelGGL((k_sgs_colour<VTF, false>), dim3(quad_blocks(cnt)), dim3(BLOCK), 0,
-			                   s, cnt, list, fac, nbr, p.d_lowmask.p, r, z);
+			hipLaunchKernelGGL((k_sgs_colour<VTF, false>), g, dim3(BLOCK), 0, s, cnt, list,
+			                   L.list, fac, sa, nbr, p.d_lowmask.p, r, sr, z, sz);
 		else
-			hipLaunchKernelGGL((k_sgs_colour<VTF, true>), dim3(quad_blocks(cnt)), dim3(BLOCK), 0,
-			                   s, cnt, list, fac, nbr, p.d_lowmask.p, r, z);
+			hipLaunchKernelGGL((k_sgs_colour<VTF, true>), g, dim3(BLOCK), 0, s, cnt, list, L.list,
+			                   fac, sa, nbr, p.d_lowmask.p, r, sr, z, sz);
 		FVX_HIP(hipGetLastError());
 	}
 	for(int c = p.ncol - 1; c >= 0; c--) {
 		const int cnt = p.coff[c+1] - p.coff[c];
 		if(cnt == 0) continue;
-		hipLaunchKernelGGL((k_sgs_colour<VTF, true>), dim3(quad_blocks(cnt)), dim3(BLOCK), 0, s,
-		                   cnt, p.d_order.p + p.coff[c], fac, nbr, p.d_lowmask.p, r, z);
+		hipLaunchKernelGGL((k_sgs_colour<VTF, true>), dim3(quad_blocks(cnt), L.nact), dim3(BLOCK),
+		                   0, s, cnt, p.d_order.p + p.coff[c], L.list, fac, sa, nbr,
+		                   p.d_lowmask.p, r, sr, z, sz);
 		FVX_HIP(hipGetLastError());
 	}
 }
 
 /// z = M^-1 r for Jacobi or ILU(0) (one application).
 template <class VTF>
-static void pc_once(const fvx_pc_s& p, const VTF* fac, const double* r, double* z,
-                    hipStream_t s)
+static void pc_once(const fvx_pc_s& p, const VTF* fac, const double* r, long sr, double* z,
+                    long sz, const Inst& L, hipStream_t s)
 {
 	const int n = p.m->nelem;
 	const int* nbr = p.m->d_nbr.p;
+	const long sa = mstride(p.m);
 	if(p.type == FVX_PC_JACOBI) {
-		hipLaunchKernelGGL(k_jacobi_apply<VTF>, dim3(quad_blocks(n)), dim3(BLOCK), 0, s, n, fac,
-		                   r, z);
+		hipLaunchKernelGGL(k_jacobi_apply<VTF>, dim3(quad_blocks(n), L.nact), dim3(BLOCK), 0, s,
+		                   n, L.list, fac, sa, r, sr, z, sz);
 		FVX_HIP(hipGetLastError());
 		return;
 	}
 	for(int c = 0; c < p.ncol; c++) {
 		const int cnt = p.coff[c+1] - p.coff[c];
 		if(cnt == 0) continue;
-		hipLaunchKernelGGL(k_ilu_fwd<VTF>, dim3(quad_blocks(cnt)), dim3(BLOCK), 0, s, cnt,
-		                   p.d_order.p + p.coff[c], fac, nbr, p.d_lowmask.p, r, z);
+		hipLaunchKernelGGL(k_ilu_fwd<VTF>, dim3(quad_blocks(cnt), L.nact), dim3(BLOCK), 0, s, cnt,
+		                   p.d_order.p + p.coff[c], L.list, fac, sa, nbr, p.d_lowmask.p, r, sr,
+		                   z, sz);
 		FVX_HIP(hipGetLastError());
 	}
 	for(int c = p.ncol - 1; c >= 0; c--) {
 		const int cnt = p.coff[c+1] - p.coff[c];
 		if(cnt == 0) continue;
-		hipLaunchKernelGGL(k_ilu_bwd<VTF>, dim3(quad_blocks(cnt)), dim3(BLOCK), 0, s, cnt,
-		                   p.d_order.p + p.coff[c], fac, nbr, p.d_lowmask.p, z);
+		hipLaunchKernelGGL(k_ilu_bwd<VTF>, dim3(quad_blocks(cnt), L.nact), dim3(BLOCK), 0, s, cnt,
+		                   p.d_order.p + p.coff[c], L.list, fac, sa, nbr, p.d_lowmask.p, z, sz);
 		FVX_HIP(hipGetLastError());
 	}
 }
@@ -710,121 +739,130 @@ static void pc_once(const fvx_pc_s& p, const VTF* fac, const double* r, double* 
 /// `sweeps` steps of z <- z + M^-1 (r - A z) from z = 0 (PETSc -pc_sor_its). Symmetric
 /// Gauss-Seidel does them as in-place passes; Jacobi and ILU(0) form the residual explicitly.
 template <class VTF>
-static void pc_apply_t(const fvx_pc_s& p, const VTF* fac, const double* r, double* z,
-                       hipStream_t s)
+static void pc_apply_t(const fvx_pc_s& p, const VTF* fac, const double* r, long sr, double* z,
+                       long sz, const Inst& L, hipStream_t s)
 {
 	if(p.type == FVX_PC_SGS) {
-		for(int k = 0; k < p.sweeps; k++) sgs_passes(p, fac, r, z, k == 0, s);
+		for(int k = 0; k < p.sweeps; k++) sgs_passes(p, fac, r, sr, z, sz, k == 0, L, s);
 		return;
 	}
-	pc_once(p, fac, r, z, s);
+	pc_once(p, fac, r, sr, z, sz, L, s);
 	if(p.sweeps <= 1) return;
 	const fvx_matrix_s& A = *p.A;
 	const int n = p.m->nelem;
 	const long N = 4L*n;
+	const dim3 g(quad_blocks(n), L.nact);
 	for(int k = 1; k < p.sweeps; k++) {
 		if(A.prec == FVX_PREC_F32)
-			hipLaunchKernelGGL(k_spmv_residual<float>, dim3(quad_blocks(n)), dim3(BLOCK), 0, s, n,
-			                   A.f32(), p.m->d_nbr.p, z, r, p.t1.p);
+			hipLaunchKernelGGL((k_spmv<float, true>), g, dim3(BLOCK), 0, s, n, L.list, A.f32(),
+			                   mstride(A.m), p.m->d_nbr.p, z, sz, r, sr, p.t1.p, N);
 		else
-			hipLaunchKernelGGL(k_spmv_residual<double>, dim3(quad_blocks(n)), dim3(BLOCK), 0, s, n,
-			                   A.f64(), p.m->d_nbr.p, z, r, p.t1.p);
+			hipLaunchKernelGGL((k_spmv<double, true>), g, dim3(BLOCK), 0, s, n, L.list, A.f64(),
+			                   mstride(A.m), p.m->d_nbr.p, z, sz, r, sr, p.t1.p, N);
 		FVX_HIP(hipGetLastError());
-		pc_once(p, fac, p.t1.p, p.t2.p, s);
-		hipLaunchKernelGGL(k_axpy1, dim3(stream_blocks(N)), dim3(BLOCK), 0, s, N, p.t2.p, z);
+		pc_once(p, fac, p.t1.p, N, p.t2.p, N, L, s);
+		hipLaunchKernelGGL(k_axpy1, dim3(stream_blocks(N), L.nact), dim3(BLOCK), 0, s, N, L.list,
+		                   p.t2.p, N, z, sz);
 		FVX_HIP(hipGetLastError());
 	}
 }
 
-void launch_pc_apply(const fvx_pc_s& p, const double* r, double* z, hipStream_t s)
+void launch_pc_apply(const fvx_pc_s& p, const double* r, long sr, double* z, long sz,
+                     const Inst& L, hipStream_t s)
 {
 	if(p.type == FVX_PC_NONE) {
-		FVX_HIP(hipMemcpyAsync(z, r, sizeof(double)*4*(size_t)p.m->nelem,
-		                       hipMemcpyDeviceToDevice, s));
+		const long N = 4L*p.m->nelem;
+		hipLaunchKernelGGL(k_scale, dim3(stream_blocks(N), L.nact), dim3(BLOCK), 0, s, N, L.list,
+		                   r, sr, p.one.p, 0L, z, sz);
+		FVX_HIP(hipGetLastError());
 		return;
 	}
 	if(!p.A) throw Error(FVX_ERR_INVALID, "preconditioner applied before its set-up");
-	if(p.prec == FVX_PREC_F32) pc_apply_t(p, p.f32(), r, z, s);
-	else pc_apply_t(p, p.f64(), r, z, s);
+	if(p.prec == FVX_PREC_F32) pc_apply_t(p, p.f32(), r, sr, z, sz, L, s);
+	else pc_apply_t(p, p.f64(), r, sr, z, sz, L, s);
 }
 
 int reduction_blocks() { return RED_BLOCKS; }
 int max_dots() { return MAX_DOTS; }
+long partial_stride() { return PART; }
 
-void launch_mdot(long N, const double* V, long ldv, int nv, const double* w, double* partial,
-                 double* out, hipStream_t s)
+void launch_mdot(long N, const double* V, long ldv, long sv, int nv, const double* w, long sw,
+                 double* partial, double* out, long so, const Inst& L, hipStream_t s)
 {
-	hipLaunchKernelGGL(k_mdot, dim3(RED_BLOCKS), dim3(BLOCK), 0, s, N, V, ldv, nv, w, partial);
-	hipLaunchKernelGGL(k_reduce, dim3(1), dim3(BLOCK), 0, s, RED_BLOCKS, partial, nv, out);
+	hipLaunchKernelGGL(k_mdot, dim3(RED_BLOCKS, L.nact), dim3(BLOCK), 0, s, N, L.list, V, ldv, sv,
+	                   nv, w, sw, partial);
+	hipLaunchKernelGGL(k_reduce, dim3(1, L.nact), dim3(BLOCK), 0, s, RED_BLOCKS, L.list, partial,
+	                   nv, out, so);
 	FVX_HIP(hipGetLastError());
 }
 
-void launch_gs_update(long N, const double* V, long ldv, int nv, const double* h, double* w,
-                      double* partial, double* nrm2, hipStream_t s)
+void launch_gs_update(long N, const double* V, long ldv, long sv, int nv, const double* h,
+                      long sh, double* w, long sw, double* partial, double* nrm2, long sn,
+                      const Inst& L, hipStream_t s)
 {
-	hipLaunchKernelGGL(k_gs_update, dim3(RED_BLOCKS), dim3(BLOCK), 0, s, N, V, ldv, nv, h, w,
-	                   partial);
-	hipLaunchKernelGGL(k_reduce, dim3(1), dim3(BLOCK), 0, s, RED_BLOCKS, partial, 1, nrm2);
+	hipLaunchKernelGGL(k_gs_update, dim3(RED_BLOCKS, L.nact), dim3(BLOCK), 0, s, N, L.list, V,
+	                   ldv, sv, nv, h, sh, w, sw, partial);
+	hipLaunchKernelGGL(k_reduce, dim3(1, L.nact), dim3(BLOCK), 0, s, RED_BLOCKS, L.list, partial,
+	                   1, nrm2, sn);
 	FVX_HIP(hipGetLastError());
 }
 
-void launch_normalise(long N, double* w, const double* nrm2, hipStream_t s)
+void launch_scale(long N, const double* b, long sb, const double* nrm2, long sn, double* out,
+                  long so, const Inst& L, hipStream_t s)
 {
-	hipLaunchKernelGGL(k_normalise, dim3(stream_blocks(N)), dim3(BLOCK), 0, s, N, w, nrm2);
+	hipLaunchKernelGGL(k_scale, dim3(stream_blocks(N), L.nact), dim3(BLOCK), 0, s, N, L.list, b,
+	                   sb, nrm2, sn, out, so);
 	FVX_HIP(hipGetLastError());
 }
 
-void launch_scale_copy(long N, const double* b, const double* nrm2, double* out, hipStream_t s)
+void launch_lincomb(long N, const double* Z, long ldz, long sz, int nv, const double* y,
+                    long sy, double* x, long sx, const Inst& L, hipStream_t s)
 {
-	hipLaunchKernelGGL(k_scale_copy, dim3(stream_blocks(N)), dim3(BLOCK), 0, s, N, b, nrm2, out);
-	FVX_HIP(hipGetLastError());
-}
-
-void launch_lincomb(long N, const double* Z, long ldz, int nv, const double* y, double* x,
-                    hipStream_t s)
-{
-	hipLaunchKernelGGL(k_lincomb, dim3(stream_blocks(N)), dim3(BLOCK), 0, s, N, Z, ldz, nv, y, x);
+	hipLaunchKernelGGL(k_lincomb, dim3(stream_blocks(N), L.nact), dim3(BLOCK), 0, s, N, L.list, Z,
+	                   ldz, sz, nv, y, sy, x, sx);
 	FVX_HIP(hipGetLastError());
 }
 
 void launch_resnorm(int n, const double* r, const double* area, double* partial, double* out,
-                    hipStream_t s)
+                    long so, const Inst& L, hipStream_t s)
 {
-	hipLaunchKernelGGL(k_resnorm, dim3(RED_BLOCKS), dim3(BLOCK), 0, s, n, r, area, partial);
-	hipLaunchKernelGGL(k_reduce, dim3(1), dim3(BLOCK), 0, s, RED_BLOCKS, partial, 1, out);
+	hipLaunchKernelGGL(k_resnorm, dim3(RED_BLOCKS, L.nact), dim3(BLOCK), 0, s, n, L.list, r,
+	                   4L*n, area, partial);
+	hipLaunchKernelGGL(k_reduce, dim3(1, L.nact), dim3(BLOCK), 0, s, RED_BLOCKS, L.list, partial,
+	                   1, out, so);
 	FVX_HIP(hipGetLastError());
 }
 
-void launch_mf_prep(long N, const double* u, const double* x, const double* xx, double eps,
-                    double* aux, hipStream_t s)
+void launch_mf_prep(long N, const double* u, const double* x, long sx, const double* xx,
+                    long sn, double eps, double* aux, const Inst& L, hipStream_t s)
 {
-	hipLaunchKernelGGL(k_mf_prep, dim3(stream_blocks(N)), dim3(BLOCK), 0, s, N, u, x, xx, eps,
-	                   aux);
+	hipLaunchKernelGGL(k_mf_prep, dim3(stream_blocks(N), L.nact), dim3(BLOCK), 0, s, N, L.list, u,
+	                   N, x, sx, xx, sn, eps, aux, N);
 	FVX_HIP(hipGetLastError());
 }
 
-void launch_mf_finish(int n, const double* x, const double* dtm, const double* ru,
-                      const double* raux, const double* xx, double eps, double* y,
-                      hipStream_t s)
+void launch_mf_finish(int n, const double* x, long sx, const double* dtm, const double* ru,
+                      const double* raux, const double* xx, long sn, double eps, double* y,
+                      long sy, const Inst& L, hipStream_t s)
 {
-	hipLaunchKernelGGL(k_mf_finish, dim3(stream_blocks(4L*n)), dim3(BLOCK), 0, s, n, x, dtm, ru,
-	                   raux, xx, eps, y);
+	hipLaunchKernelGGL(k_mf_finish, dim3(stream_blocks(4L*n), L.nact), dim3(BLOCK), 0, s, n,
+	                   L.list, x, sx, dtm, ru, raux, xx, sn, eps, y, sy);
 	FVX_HIP(hipGetLastError());
 }
 
 void launch_update_implicit(int n, double* u, const double* du, int full, double min_relax,
-                            double gamma, hipStream_t s)
+                            double gamma, const Inst& L, hipStream_t s)
 {
-	hipLaunchKernelGGL(k_update_implicit, dim3(thread_blocks(n)), dim3(BLOCK), 0, s, n, u, du,
-	                   full, min_relax, gamma);
+	hipLaunchKernelGGL(k_update_implicit, dim3(thread_blocks(n), L.nact), dim3(BLOCK), 0, s, n,
+	                   L.list, u, du, full, min_relax, gamma);
 	FVX_HIP(hipGetLastError());
 }
 
 void launch_update_explicit(int n, double* u, const double* r, const double* dt,
-                            const double* area, double cfl, hipStream_t s)
+                            const double* area, double cfl, const Inst& L, hipStream_t s)
 {
-	hipLaunchKernelGGL(k_update_explicit, dim3(thread_blocks(n)), dim3(BLOCK), 0, s, n, u, r, dt,
-	                   area, cfl);
+	hipLaunchKernelGGL(k_update_explicit, dim3(thread_blocks(n), L.nact), dim3(BLOCK), 0, s, n,
+	                   L.list, u, r, dt, area, cfl);
 	FVX_HIP(hipGetLastError());
 }
 

@@ -295,11 +295,9 @@ void rcm_order(const fvx_mesh_desc& d, int* perm)
  *  - Green-Gauss with inverse-distance face values (:61-214):
  *        c_ij = len_f n_out,f (d_j/(d_i+d_j)) / A_i,  d = 1/|x_f - x|
  *    (the u_i term carries sum_f len_f n_out,f = 0 for a closed cell). */
-void build_disc(fvx_disc_s& d)
+GasParams make_gas(const fvx_physics_cfg& p)
 {
-	const fvx_mesh_s& m = *d.m;
-	const fvx_physics_cfg& p = d.pc;
-	GasParams& g = d.gas;
+	GasParams g;
 	g.g = p.gamma; g.gm1 = p.gamma - 1.0;
 	g.Minf = p.Minf; g.Tinf = p.Tinf; g.Reinf = p.Reinf; g.Pr = p.Pr;
 	g.gM2 = p.gamma*p.Minf*p.Minf;
@@ -310,6 +308,14 @@ void build_disc(fvx_disc_s& d)
 	g.uinf[2] = std::sin(p.aoa);
 	g.uinf[3] = g.pinf/(p.gamma - 1.0) + 0.5;
 	g.constvisc = p.const_visc;
+	return g;
+}
+
+void build_disc(fvx_disc_s& d)
+{
+	const fvx_mesh_s& m = *d.m;
+	const fvx_physics_cfg& p = d.pc;
+	d.gas = make_gas(p);
 
 	if(p.nbc < 0 || p.nbc > 8) throw Error(FVX_ERR_INVALID, "at most 8 boundary conditions");
 	for(int k = 0; k < 8; k++) { d.bct.kind[k] = -1; d.bct.v0[k] = 0; d.bct.v1[k] = 0; }
@@ -325,6 +331,8 @@ void build_disc(fvx_disc_s& d)
 		if(d.bcid[b] < 0) throw Error(FVX_ERR_INVALID, "boundary marker without a boundary condition");
 	}
 	d.d_bcid.upload(d.bcid);
+	d.d_gas.upload(std::vector<GasParams>(1, d.gas));
+	d.d_bct.upload(std::vector<BCTable>(1, d.bct));
 
 	const fvx_numerics_cfg& n = d.nc;
 	if(n.conv_numflux < 0 || n.conv_numflux > 6 || n.conv_numflux_jac < 0 || n.conv_numflux_jac > 6)

@@ -1,8 +1,13 @@
 /* solver.cpp -- host side of the implicit solver: matrix and preconditioner set-up, the
- * flexible GMRES driver, the matrix-free operator and the steady pseudo-time loop. Every
- * vector operation runs on the device; the host only orders launches, keeps the small
- * Hessenberg system of GMRES and the CFL schedule, and reads one scalar per Krylov iteration
- * and per pseudo-time step.
+ * flexible GMRES driver, the matrix-free operator and the steady pseudo-time loop.
+ *
+ * Everything is written for a set of instances (operating points on one mesh) solved
+ * together: each launch covers all active instances, while every instance keeps its own
+ * Krylov basis, Hessenberg system, CFL ramp and stopping decisions, so an instance evolves
+ * exactly as it would alone. A single solve is the one-instance case (no instance list, the
+ * discretisation's own parameters). All vector work runs on the device; the host keeps the
+ * small Hessenberg systems and the CFL schedules and reads a few scalars per instance per
+ * Krylov iteration and per pseudo-time step.
  */
 #include "fvx_internal.hpp"
 #include <algorithm>
@@ -15,13 +20,14 @@ namespace fvx {
 
 /* ---------------------------------------------------------------- matrix --------------- */
 
-void matrix_init(fvx_matrix_s& A, const fvx_mesh_s* m, int prec)
+void matrix_init(fvx_matrix_s& A, const fvx_mesh_s* m, int prec, int ninst)
 {
 	if(prec != FVX_PREC_F64 && prec != FVX_PREC_F32)
 		throw Error(FVX_ERR_INVALID, "unknown matrix precision");
 	A.m = m;
 	A.prec = prec;
-	const size_t cnt = (size_t)m->nelem*80;
+	A.ninst = ninst;
+	const size_t cnt = (size_t)m->nelem*80*ninst;
 	if(prec == FVX_PREC_F32) {
 		A.v32.alloc(cnt);
 		FVX_HIP(hipMemset(A.v32.p, 0, cnt*sizeof(float)));
@@ -73,7 +79,7 @@ void matrix_export_bsr(const fvx_matrix_s& A, int* rowptr, int* colind, double* 
 
 /* ------------------------------------------------------------ preconditioner ----------- */
 
-void pc_init(fvx_pc_s& p, const fvx_mesh_s* m, int type, int prec)
+void pc_init(fvx_pc_s& p, const fvx_mesh_s* m, int type, int prec, int ninst)
 {
 	if(type < FVX_PC_NONE || type > FVX_PC_ILU0)
 		throw Error(FVX_ERR_INVALID, "unknown preconditioner type");
@@ -82,6 +88,8 @@ void pc_init(fvx_pc_s& p, const fvx_mesh_s* m, int type, int prec)
 	p.m = m;
 	p.type = type;
 	p.prec = prec;
+	p.ninst = ninst;
+	p.A = nullptr;
 	const int n = m->nelem;
 	// greedy colouring in cell order: the smallest colour unused by lower-numbered neighbours
 	p.colour.assign(n, 0);
@@ -116,8 +124,9 @@ void pc_init(fvx_pc_s& p, const fvx_mesh_s* m, int type, int prec)
 		p.d_order.upload(p.order);
 		p.d_lowmask.upload(low);
 	}
+	p.one.upload(std::vector<double>(1, 1.0));
 	if(type != FVX_PC_NONE) {
-		const size_t cnt = (size_t)n*80;
+		const size_t cnt = (size_t)n*80*ninst;
 		if(prec == FVX_PREC_F32) {
 			p.v32.alloc(cnt);
 			FVX_HIP(hipMemset(p.v32.p, 0, cnt*sizeof(float)));
@@ -126,6 +135,9 @@ void pc_init(fvx_pc_s& p, const fvx_mesh_s* m, int type, int prec)
 			FVX_HIP(hipMemset(p.v64.p, 0, cnt*sizeof(double)));
 		}
 	}
+	p.t1.release();
+	p.t2.release();
+	pc_set_sweeps(p, p.sweeps);
 }
 
 void pc_set_sweeps(fvx_pc_s& p, int sweeps)
@@ -133,100 +145,174 @@ void pc_set_sweeps(fvx_pc_s& p, int sweeps)
 	if(sweeps < 1 || sweeps > 64) throw Error(FVX_ERR_INVALID, "sweep count must be in [1, 64]");
 	p.sweeps = sweeps;
 	if(sweeps > 1 && p.type != FVX_PC_SGS && p.type != FVX_PC_NONE) {
-		const size_t N = 4*(size_t)p.m->nelem;
+		const size_t N = 4*(size_t)p.m->nelem*p.ninst;
 		if(p.t1.n != N) { p.t1.alloc(N); p.t2.alloc(N); }
 	}
 }
 
 /* ---------------------------------------------------------------- Krylov --------------- */
 
-/// Device storage of one FGMRES cycle and the pinned host slots for its scalars.
+using Op = std::function<void(const double* x, long sx, double* y, long sy, const Inst& L)>;
+
+/// Device storage of one FGMRES cycle for up to `cap` instances, per-instance scalar slots
+/// (h column, norms, solution weights) and their pinned host mirror.
 struct Krylov {
 	long N = 0;
-	int m = 0;
+	int m = 0, cap = 0;
 	DBuf<double> V, Z, partial, scal;
+	DBuf<int> dlist;
 	double* hpin = nullptr;
+	int* ipin = nullptr;
 	static constexpr int H0 = 0, NRM = 34, BETA = 35, XX = 36, RES = 37, Y0 = 40, NSCAL = 80;
 
-	void ensure(long n_, int m_)
+	void ensure(long n_, int m_, int cap_)
 	{
-		if(!hpin) FVX_HIP(hipHostMalloc((void**)&hpin, NSCAL*sizeof(double)));
-		if(n_ == N && m_ <= m) return;
+		if(n_ == N && m_ <= m && cap_ <= cap) return;
+		release_pinned();
 		N = n_;
 		m = m_;
-		V.alloc((size_t)(m + 1)*N);
-		Z.alloc((size_t)m*N);
-		partial.alloc((size_t)reduction_blocks()*max_dots());
-		scal.alloc(NSCAL);
+		cap = cap_;
+		V.alloc((size_t)cap*(m + 1)*N);
+		Z.alloc((size_t)cap*m*N);
+		partial.alloc((size_t)cap*partial_stride());
+		scal.alloc((size_t)cap*NSCAL);
+		dlist.alloc(cap);
+		FVX_HIP(hipHostMalloc((void**)&hpin, (size_t)cap*NSCAL*sizeof(double)));
+		FVX_HIP(hipHostMalloc((void**)&ipin, (size_t)cap*sizeof(int)));
+		std::memset(hpin, 0, (size_t)cap*NSCAL*sizeof(double));
 	}
-	~Krylov() { if(hpin) (void)hipHostFree(hpin); }
-
-	void fetch(int first, int count, hipStream_t s)
+	void release_pinned()
 	{
-		FVX_HIP(hipMemcpyAsync(hpin + first, scal.p + first, count*sizeof(double),
+		if(hpin) (void)hipHostFree(hpin);
+		if(ipin) (void)hipHostFree(ipin);
+		hpin = nullptr;
+		ipin = nullptr;
+	}
+	~Krylov() { release_pinned(); }
+
+	long sV() const { return (long)(m + 1)*N; }
+	long sZ() const { return (long)m*N; }
+	double& h(int i, int slot) { return hpin[(size_t)i*NSCAL + slot]; }
+
+	/// Launch description for the instances `ids`: no list for a single-instance solve,
+	/// otherwise the list uploaded to the device (the upload is waited for, so the pinned
+	/// staging buffer can be reused at once).
+	Inst inst(const std::vector<int>& ids, bool single, hipStream_t s)
+	{
+		if(single) return Inst{nullptr, 1};
+		std::memcpy(ipin, ids.data(), ids.size()*sizeof(int));
+		FVX_HIP(hipMemcpyAsync(dlist.p, ipin, ids.size()*sizeof(int), hipMemcpyHostToDevice, s));
+		FVX_HIP(hipStreamSynchronize(s));
+		return Inst{dlist.p, (int)ids.size()};
+	}
+
+	/// Device scalar slots of every instance to the host (one copy, then waits).
+	void fetch(hipStream_t s)
+	{
+		FVX_HIP(hipMemcpyAsync(hpin, scal.p, (size_t)cap*NSCAL*sizeof(double),
 		                       hipMemcpyDeviceToHost, s));
 		FVX_HIP(hipStreamSynchronize(s));
 	}
 
-	using Op = std::function<void(const double*, double*)>;
-
-	/// One cycle of right-preconditioned FGMRES from x = 0 with classical Gram-Schmidt.
-	int solve(const Op& op, const Op& pc, const double* b, double* x, int mm, double rtol,
-	          hipStream_t s)
+	/// One cycle of right-preconditioned FGMRES from x = 0 with classical Gram-Schmidt for the
+	/// instances `ids` (instance 0 alone when `single`). b and x are instance-major with
+	/// stride N. Returns the iteration count per instance in `its`, or -1 where a non-finite
+	/// value appeared (that instance's x is then undefined).
+	void solve(const Op& op, const Op& pc, const double* b, double* x, int mm, double rtol,
+	           const std::vector<int>& ids, bool single, std::vector<int>& its, hipStream_t s)
 	{
-		double* d_h = scal.p + H0;
-		launch_mdot(N, b, 0, 1, b, partial.p, scal.p + BETA, s);
-		fetch(BETA, 1, s);
-		const double beta = std::sqrt(hpin[BETA]);
-		if(!std::isfinite(beta)) throw Error(FVX_ERR_NONFINITE, "non-finite right-hand side");
-		if(beta == 0.0) {
-			FVX_HIP(hipMemsetAsync(x, 0, N*sizeof(double), s));
-			return 0;
+		its.assign(cap, 0);
+		Inst L = inst(ids, single, s);
+		launch_mdot(N, b, 0, N, 1, b, N, partial.p, scal.p + BETA, NSCAL, L, s);
+		fetch(s);
+		struct St { std::vector<double> H, cs, sn, g; double beta = 0; int it = 0; };
+		std::vector<St> st(cap);
+		std::vector<int> act;
+		for(int i : ids) {
+			const double beta = std::sqrt(h(i, BETA));
+			if(!std::isfinite(beta)) { its[i] = -1; continue; }
+			if(beta == 0.0) {
+				FVX_HIP(hipMemsetAsync(x + (size_t)i*N, 0, N*sizeof(double), s));
+				continue;
+			}
+			St& t = st[i];
+			t.H.assign((size_t)(mm + 1)*mm, 0.0);
+			t.cs.assign(mm, 0.0); t.sn.assign(mm, 0.0); t.g.assign(mm + 1, 0.0);
+			t.beta = beta; t.g[0] = beta;
+			act.push_back(i);
 		}
-		launch_scale_copy(N, b, scal.p + BETA, V.p, s);
-		std::vector<double> H((size_t)(mm + 1)*mm, 0.0), cs(mm), sn(mm), g(mm + 1, 0.0);
-		g[0] = beta;
-		int it = 0;
-		for(int j = 0; j < mm; j++) {
+		if(act.empty()) { FVX_HIP(hipStreamSynchronize(s)); return; }
+		const std::vector<int> started = act;
+		if(act.size() != ids.size()) L = inst(act, single, s);
+		launch_scale(N, b, N, scal.p + BETA, NSCAL, V.p, sV(), L, s);
+		for(int j = 0; j < mm && !act.empty(); j++) {
 			double* vj = V.p + (size_t)j*N;
 			double* zj = Z.p + (size_t)j*N;
 			double* w = V.p + (size_t)(j + 1)*N;
-			pc(vj, zj);
-			op(zj, w);
-			launch_mdot(N, V.p, N, j + 1, w, partial.p, d_h, s);
-			launch_gs_update(N, V.p, N, j + 1, d_h, w, partial.p, scal.p + NRM, s);
-			fetch(H0, NRM + 1, s);
-			for(int i = 0; i <= j; i++) H[(size_t)i*mm + j] = hpin[H0 + i];
-			const double hn = std::sqrt(hpin[NRM]);
-			if(!std::isfinite(hn)) throw Error(FVX_ERR_NONFINITE, "non-finite Krylov vector");
-			H[(size_t)(j + 1)*mm + j] = hn;
-			if(hn != 0.0) launch_normalise(N, w, scal.p + NRM, s);
-			for(int i = 0; i < j; i++) {
-				const double t = cs[i]*H[(size_t)i*mm + j] + sn[i]*H[(size_t)(i + 1)*mm + j];
-				H[(size_t)(i + 1)*mm + j] = -sn[i]*H[(size_t)i*mm + j] + cs[i]*H[(size_t)(i + 1)*mm + j];
-				H[(size_t)i*mm + j] = t;
+			pc(vj, sV(), zj, sZ(), L);
+			op(zj, sZ(), w, sV(), L);
+			launch_mdot(N, V.p, N, sV(), j + 1, w, sV(), partial.p, scal.p + H0, NSCAL, L, s);
+			launch_gs_update(N, V.p, N, sV(), j + 1, scal.p + H0, NSCAL, w, sV(), partial.p,
+			                 scal.p + NRM, NSCAL, L, s);
+			// normalise v_{j+1}; after an exact breakdown (norm 0) the vector is never used
+			launch_scale(N, w, sV(), scal.p + NRM, NSCAL, w, sV(), L, s);
+			fetch(s);
+			std::vector<int> next;
+			for(int i : act) {
+				St& t = st[i];
+				double* H = t.H.data();
+				for(int r = 0; r <= j; r++) H[(size_t)r*mm + j] = h(i, H0 + r);
+				const double hn = std::sqrt(h(i, NRM));
+				if(!std::isfinite(hn)) { its[i] = -1; continue; }
+				H[(size_t)(j + 1)*mm + j] = hn;
+				for(int r = 0; r < j; r++) {
+					const double a = t.cs[r]*H[(size_t)r*mm + j] + t.sn[r]*H[(size_t)(r + 1)*mm + j];
+					H[(size_t)(r + 1)*mm + j] = -t.sn[r]*H[(size_t)r*mm + j]
+					                            + t.cs[r]*H[(size_t)(r + 1)*mm + j];
+					H[(size_t)r*mm + j] = a;
+				}
+				const double a = H[(size_t)j*mm + j], c = H[(size_t)(j + 1)*mm + j];
+				const double den = std::sqrt(a*a + c*c);
+				t.cs[j] = a/den;
+				t.sn[j] = c/den;
+				H[(size_t)j*mm + j] = den;
+				H[(size_t)(j + 1)*mm + j] = 0.0;
+				t.g[j + 1] = -t.sn[j]*t.g[j];
+				t.g[j] = t.cs[j]*t.g[j];
+				t.it = j + 1;
+				if(!(std::fabs(t.g[j + 1]) <= rtol*t.beta || hn == 0.0)) next.push_back(i);
 			}
-			const double a = H[(size_t)j*mm + j], c = H[(size_t)(j + 1)*mm + j];
-			const double den = std::sqrt(a*a + c*c);
-			cs[j] = a/den;
-			sn[j] = c/den;
-			H[(size_t)j*mm + j] = den;
-			H[(size_t)(j + 1)*mm + j] = 0.0;
-			g[j + 1] = -sn[j]*g[j];
-			g[j] = cs[j]*g[j];
-			it = j + 1;
-			if(std::fabs(g[j + 1]) <= rtol*beta || hn == 0.0) break;
+			if(!next.empty() && next.size() != act.size()) L = inst(next, single, s);
+			act.swap(next);
 		}
-		for(int i = it - 1; i >= 0; i--) {
-			double t = g[i];
-			for(int k = i + 1; k < it; k++) t -= H[(size_t)i*mm + k]*hpin[Y0 + k];
-			hpin[Y0 + i] = t/H[(size_t)i*mm + i];
+		// solution weights, then x = sum_k y_k Z_k for the instances with equal counts together
+		std::vector<int> done;
+		for(int i : started) {
+			if(its[i] < 0) continue;
+			St& t = st[i];
+			for(int r = t.it - 1; r >= 0; r--) {
+				double a = t.g[r];
+				for(int k = r + 1; k < t.it; k++) a -= t.H[(size_t)r*mm + k]*h(i, Y0 + k);
+				h(i, Y0 + r) = a/t.H[(size_t)r*mm + r];
+			}
+			its[i] = t.it;
+			done.push_back(i);
 		}
-		FVX_HIP(hipMemcpyAsync(scal.p + Y0, hpin + Y0, it*sizeof(double),
+		if(done.empty()) { FVX_HIP(hipStreamSynchronize(s)); return; }
+		FVX_HIP(hipMemcpyAsync(scal.p, hpin, (size_t)cap*NSCAL*sizeof(double),
 		                       hipMemcpyHostToDevice, s));
-		launch_lincomb(N, Z.p, N, it, scal.p + Y0, x, s);
+		std::sort(done.begin(), done.end(), [&](int a, int b) {
+			return its[a] != its[b] ? its[a] < its[b] : a < b; });
+		size_t k0 = 0;
+		while(k0 < done.size()) {
+			size_t k1 = k0;
+			while(k1 < done.size() && its[done[k1]] == its[done[k0]]) k1++;
+			const std::vector<int> grp(done.begin() + k0, done.begin() + k1);
+			const Inst G = inst(grp, single, s);
+			launch_lincomb(N, Z.p, N, sZ(), its[grp[0]], scal.p + Y0, NSCAL, x, N, G, s);
+			k0 = k1;
+		}
 		FVX_HIP(hipStreamSynchronize(s));
-		return it;
 	}
 };
 
@@ -241,33 +327,53 @@ int linear_solve(const fvx_matrix_s& A, const fvx_pc_s& p, const double* b, doub
 {
 	check_restart(restart);
 	Krylov k;
-	k.ensure(4L*A.m->nelem, restart);
-	return k.solve([&](const double* xi, double* yo) { launch_spmv(A, xi, yo, s); },
-	               [&](const double* r, double* z) { launch_pc_apply(p, r, z, s); },
-	               b, x, restart, rtol, s);
+	k.ensure(4L*A.m->nelem, restart, 1);
+	std::vector<int> its;
+	k.solve([&](const double* xi, long sx, double* yo, long sy, const Inst& L) {
+	            launch_spmv(A, xi, sx, yo, sy, L, s); },
+	        [&](const double* r, long sr, double* z, long sz, const Inst& L) {
+	            launch_pc_apply(p, r, sr, z, sz, L, s); },
+	        b, x, restart, rtol, {0}, true, its, s);
+	if(its[0] < 0) throw Error(FVX_ERR_NONFINITE, "non-finite value in the Krylov solve");
+	return its[0];
+}
+
+/// y = dtm x + (r(u + eps x/|x|) - r(u)) |x|/eps for the instances of L; raux holds
+/// raux_len values (all instances) and is cleared before the residual is accumulated.
+static void matfree_batch(const fvx_disc_s& d, const double* u, const double* ru,
+                          const double* dtm, const double* x, long sx, double* y, long sy,
+                          double eps, double* aux, double* raux, size_t raux_len,
+                          double* partial, double* xx, long sxx, const Inst& L,
+                          const BatchArg& ba, double* grad, double* rgrad, hipStream_t s)
+{
+	const int n = d.m->nelem;
+	const long N = 4L*n;
+	launch_mdot(N, x, 0, sx, 1, x, sx, partial, xx, sxx, L, s);
+	launch_mf_prep(N, u, x, sx, xx, sxx, eps, aux, L, s);
+	FVX_HIP(hipMemsetAsync(raux, 0, raux_len*sizeof(double), s));
+	launch_residual(d, aux, raux, 0, nullptr, s, ba, L.nact, grad, rgrad);
+	launch_mf_finish(n, x, sx, dtm, ru, raux, xx, sxx, eps, y, sy, L, s);
 }
 
 void matfree_apply(const fvx_disc_s& d, const double* u, const double* ru, const double* dtm,
                    const double* x, double* y, double eps, double* aux, double* raux,
                    double* partial, double* xx, hipStream_t s)
 {
-	const int n = d.m->nelem;
-	const long N = 4L*n;
-	launch_mdot(N, x, 0, 1, x, partial, xx, s);
-	launch_mf_prep(N, u, x, xx, eps, aux, s);
-	FVX_HIP(hipMemsetAsync(raux, 0, N*sizeof(double), s));
-	launch_residual(d, aux, raux, 0, nullptr, s);
-	launch_mf_finish(n, x, dtm, ru, raux, xx, eps, y, s);
+	const long N = 4L*d.m->nelem;
+	matfree_batch(d, u, ru, dtm, x, N, y, N, eps, aux, raux, N, partial, xx, 0, Inst(),
+	              BatchArg(), nullptr, nullptr, s);
 }
 
 /* ------------------------------------------------------------ steady solver ------------ */
 
 struct SolverWork {
-	int n = -1, jprec = -1, pctype = -1, pcprec = -1;
+	int n = -1, ninst = 0, jprec = -1, pctype = -1, pcprec = -1;
 	fvx_matrix_s A;
 	fvx_pc_s P;
-	DBuf<double> r, dt, dtm, du, aux, raux;
+	DBuf<double> r, dt, dtm, du, aux, raux, grad, rgrad, cfl;
+	double* cfl_pin = nullptr;
 	Krylov kry;
+	~SolverWork() { if(cfl_pin) (void)hipHostFree(cfl_pin); }
 };
 
 void destroy_solver_work(SolverWork* w) { delete w; }
@@ -282,15 +388,10 @@ static double exp_ramp(double cmin, double cmax, double prev, double ratio, doub
 	return nc;
 }
 
-void steady_solve(fvx_disc_s& d, const fvx_solver_cfg& c, double* u, fvx_solve_stats& st,
-                  hipStream_t s)
+static void validate(const fvx_disc_s& d, const fvx_solver_cfg& c)
 {
-	std::memset(&st, 0, sizeof(st));
 	if(!(c.tol > 0) || c.maxiter < 0 || !(c.cflinit > 0) || !(c.cflfin >= c.cflinit))
 		throw Error(FVX_ERR_INVALID, "invalid pseudo-time settings");
-	const fvx_mesh_s& m = *d.m;
-	const int n = m.nelem;
-	const long N = 4L*n;
 	if(c.implicit) {
 		check_restart(c.krylov_restart);
 		if(!jacobian_supported(d))
@@ -298,89 +399,176 @@ void steady_solve(fvx_disc_s& d, const fvx_solver_cfg& c, double* u, fvx_solve_s
 		if(c.matrix_free && !(c.mf_eps > 0))
 			throw Error(FVX_ERR_INVALID, "matrix-free step must be positive");
 	}
+}
+
+void steady_core(fvx_disc_s& d, const fvx_solver_cfg& c, double* u, int ninst,
+                 const GasParams* gp, std::vector<fvx_solve_stats>& stats, hipStream_t s)
+{
+	validate(d, c);
+	const fvx_mesh_s& m = *d.m;
+	const int n = m.nelem;
+	const long N = 4L*n;
+	const bool single = gp == nullptr;
+	if(single && ninst != 1) throw Error(FVX_ERR_INVALID, "a single solve has one instance");
 	if(!d.work) d.work = new SolverWork;
 	SolverWork& w = *d.work;
-	if(w.n != n) {
-		w.r.alloc(N); w.dt.alloc(n); w.dtm.alloc(n); w.du.alloc(N);
-		w.aux.alloc(N); w.raux.alloc(N);
+	if(w.n != n || w.ninst != ninst) {
+		w.r.alloc(N*ninst); w.dt.alloc((size_t)n*ninst); w.dtm.alloc((size_t)n*ninst);
+		w.du.alloc(N*ninst); w.aux.alloc(N*ninst); w.raux.alloc(N*ninst);
+		if(!single) { w.grad.alloc(8L*n*ninst); w.rgrad.alloc(8L*n*ninst); }
+		w.cfl.alloc(ninst);
+		if(w.cfl_pin) (void)hipHostFree(w.cfl_pin);
+		w.cfl_pin = nullptr;
+		FVX_HIP(hipHostMalloc((void**)&w.cfl_pin, ninst*sizeof(double)));
 		w.n = n;
+		w.ninst = ninst;
 		w.jprec = w.pctype = w.pcprec = -1;
 	}
 	if(c.implicit) {
-		if(w.jprec != c.jac_precision) { matrix_init(w.A, &m, c.jac_precision); w.jprec = c.jac_precision; }
+		if(w.jprec != c.jac_precision) {
+			matrix_init(w.A, &m, c.jac_precision, ninst);
+			w.jprec = c.jac_precision;
+		}
 		if(w.pctype != c.pc || w.pcprec != c.pc_precision) {
-			pc_init(w.P, &m, c.pc, c.pc_precision);
+			pc_init(w.P, &m, c.pc, c.pc_precision, ninst);
 			w.pctype = c.pc;
 			w.pcprec = c.pc_precision;
 		}
 		pc_set_sweeps(w.P, c.pc_sweeps > 0 ? c.pc_sweeps : 1);
-		w.kry.ensure(N, c.krylov_restart);
 	}
 	Krylov& K = w.kry;
-	if(!K.hpin) K.ensure(N, 1);
+	K.ensure(N, c.implicit ? c.krylov_restart : 1, ninst);
+	double* grad = single ? nullptr : w.grad.p;
+	double* rgrad = single ? nullptr : w.rgrad.p;
 
-	double cfl = 0.0, resi = 1.0, resiold = 1.0;
-	double initres = c.fixed_norm > 0 ? c.fixed_norm : 1.0;
-	int step = 0;
+	struct Ist { double cfl = 0, resi = 1, resiold = 1, initres = 1; int step = 0; };
+	std::vector<Ist> is(ninst);
+	stats.assign(ninst, fvx_solve_stats{});
+	for(auto& t : is) t.initres = c.fixed_norm > 0 ? c.fixed_norm : 1.0;
+	auto active = [&](int i) {
+		return stats[i].status == 0 && is[i].resi/is[i].initres > c.tol && is[i].step < c.maxiter;
+	};
 	FVX_HIP(hipStreamSynchronize(s));
 	const auto t0 = std::chrono::steady_clock::now();
-	while(resi/initres > c.tol && step < c.maxiter) {
-		FVX_HIP(hipMemsetAsync(w.r.p, 0, N*sizeof(double), s));
-		launch_residual(d, u, w.r.p, 1, w.dt.p, s);
-		launch_resnorm(n, w.r.p, m.d_area.p, K.partial.p, K.scal.p + Krylov::RES, s);
+	auto now = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+	std::vector<double> t_end(ninst, 0.0);
+	std::vector<int> its;
+	for(;;) {
+		std::vector<int> act;
+		for(int i = 0; i < ninst; i++) if(active(i)) act.push_back(i);
+		if(act.empty()) break;
+		Inst L = K.inst(act, single, s);
+		BatchArg ba;
+		if(!single) { ba.list = L.list; ba.gp = gp; ba.cfl = w.cfl.p; }
+		FVX_HIP(hipMemsetAsync(w.r.p, 0, (size_t)N*ninst*sizeof(double), s));
+		launch_residual(d, u, w.r.p, 1, w.dt.p, s, ba, L.nact, grad, rgrad);
+		launch_resnorm(n, w.r.p, m.d_area.p, K.partial.p, K.scal.p + Krylov::RES, Krylov::NSCAL,
+		               L, s);
 		if(c.implicit) {
-			cfl = exp_ramp(c.cflinit, c.cflfin, cfl, resiold/resi, 0.25, 0.3);
-			launch_jacobian(d, u, w.A.raw(), w.A.prec, w.dt.p, cfl, w.dtm.p, s);
+			for(int i : act) {
+				is[i].cfl = exp_ramp(c.cflinit, c.cflfin, is[i].cfl, is[i].resiold/is[i].resi,
+				                     0.25, 0.3);
+				w.cfl_pin[i] = is[i].cfl;
+			}
+			if(!single)
+				FVX_HIP(hipMemcpyAsync(w.cfl.p, w.cfl_pin, ninst*sizeof(double),
+				                       hipMemcpyHostToDevice, s));
+			launch_jacobian(d, u, w.A.raw(), w.A.prec, w.dt.p, is[act[0]].cfl, w.dtm.p, s, ba,
+			                L.nact);
 			d.prof.start(PROF_PC, s);
-			launch_pc_setup(w.P, w.A, s);
+			launch_pc_setup(w.P, w.A, L, s);
 			d.prof.stop(s);
-			Krylov::Op op;
+			K.fetch(s);
+			// an instance whose residual norm is not finite stops here; the others go on
+			std::vector<int> ok;
+			for(int i : act) {
+				const double rn2 = K.h(i, Krylov::RES);
+				if(!std::isfinite(rn2)) {
+					is[i].resiold = is[i].resi;
+					is[i].resi = std::sqrt(rn2);
+					stats[i].status = FVX_ERR_NONFINITE;
+					t_end[i] = now();
+				} else ok.push_back(i);
+			}
+			if(ok.empty()) continue;
+			if(ok.size() != act.size()) {
+				act = ok;
+				L = K.inst(act, single, s);
+				ba.list = L.list;
+			}
+			Op op;
 			if(c.matrix_free)
-				op = [&](const double* x, double* y) {
-					matfree_apply(d, u, w.r.p, w.dtm.p, x, y, c.mf_eps, w.aux.p, w.raux.p,
-					              K.partial.p, K.scal.p + Krylov::XX, s);
+				op = [&](const double* x, long sx, double* y, long sy, const Inst& LL) {
+					BatchArg bb = ba;
+					bb.list = LL.list;
+					matfree_batch(d, u, w.r.p, w.dtm.p, x, sx, y, sy, c.mf_eps, w.aux.p, w.raux.p,
+					              (size_t)N*ninst, K.partial.p, K.scal.p + Krylov::XX,
+					              Krylov::NSCAL, LL, bb, grad, rgrad, s);
 				};
 			else
-				op = [&](const double* x, double* y) { launch_spmv(w.A, x, y, s); };
-			// the residual norm was reduced before the Krylov solve reuses the partials
-			FVX_HIP(hipMemcpyAsync(K.hpin + Krylov::RES, K.scal.p + Krylov::RES,
-			                       sizeof(double), hipMemcpyDeviceToHost, s));
-			FVX_HIP(hipStreamSynchronize(s));
-			const double rn2 = K.hpin[Krylov::RES];
-			if(!std::isfinite(rn2)) {
-				resiold = resi;
-				resi = std::sqrt(rn2);
-				st.status = FVX_ERR_NONFINITE;
-				break;
+				op = [&](const double* x, long sx, double* y, long sy, const Inst& LL) {
+					launch_spmv(w.A, x, sx, y, sy, LL, s);
+				};
+			std::vector<double> rn2(ninst);
+			for(int i : act) rn2[i] = K.h(i, Krylov::RES);
+			K.solve(op, [&](const double* r, long sr, double* z, long sz, const Inst& LL) {
+			            launch_pc_apply(w.P, r, sr, z, sz, LL, s); },
+			        w.r.p, w.du.p, c.krylov_restart, c.krylov_rtol, act, single, its, s);
+			std::vector<int> upd;
+			for(int i : act) {
+				if(its[i] < 0) { stats[i].status = FVX_ERR_NONFINITE; t_end[i] = now(); continue; }
+				stats[i].lin_iters += its[i];
+				upd.push_back(i);
 			}
-			const int its = K.solve(op, [&](const double* r, double* z) {
-				                        launch_pc_apply(w.P, r, z, s); },
-			                        w.r.p, w.du.p, c.krylov_restart, c.krylov_rtol, s);
-			st.lin_iters += its;
-			launch_update_implicit(n, u, w.du.p, c.fullupdate, c.min_relax, d.gas.g, s);
-			resiold = resi;
-			resi = std::sqrt(rn2);
+			if(!upd.empty()) {
+				const Inst U = upd.size() == act.size() ? L : K.inst(upd, single, s);
+				launch_update_implicit(n, u, w.du.p, c.fullupdate, c.min_relax, d.gas.g, U, s);
+			}
+			for(int i : upd) {
+				is[i].resiold = is[i].resi;
+				is[i].resi = std::sqrt(rn2[i]);
+			}
+			act = upd;
 		} else {
-			cfl = exp_ramp(c.cflinit, c.cflfin, cfl, resiold/resi, 0.3, 0.25);
-			launch_update_explicit(n, u, w.r.p, w.dt.p, m.d_area.p, c.cflinit, s);
-			FVX_HIP(hipMemcpyAsync(K.hpin + Krylov::RES, K.scal.p + Krylov::RES,
-			                       sizeof(double), hipMemcpyDeviceToHost, s));
-			FVX_HIP(hipStreamSynchronize(s));
-			resiold = resi;
-			resi = std::sqrt(K.hpin[Krylov::RES]);
+			for(int i : act)
+				is[i].cfl = exp_ramp(c.cflinit, c.cflfin, is[i].cfl, is[i].resiold/is[i].resi,
+				                     0.3, 0.25);
+			launch_update_explicit(n, u, w.r.p, w.dt.p, m.d_area.p, c.cflinit, L, s);
+			K.fetch(s);
+			for(int i : act) {
+				is[i].resiold = is[i].resi;
+				is[i].resi = std::sqrt(K.h(i, Krylov::RES));
+			}
 		}
-		if(!std::isfinite(resi)) { st.status = FVX_ERR_NONFINITE; break; }
-		if(step == 0 && c.fixed_norm <= 0) initres = resi;
-		step++;
+		for(int i : act) {
+			if(!std::isfinite(is[i].resi)) stats[i].status = FVX_ERR_NONFINITE;
+			else {
+				if(is[i].step == 0 && c.fixed_norm <= 0) is[i].initres = is[i].resi;
+				is[i].step++;
+			}
+			if(!active(i)) t_end[i] = now();
+		}
 	}
 	FVX_HIP(hipStreamSynchronize(s));
-	st.walltime = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-	st.steps = step;
-	st.final_abs = resi;
-	st.first_norm = initres;
-	st.final_rel = resi/initres;
-	st.converged = (st.status == 0 && resi/initres <= c.tol) ? 1 : 0;
-	if(st.status == 0 && !st.converged) st.status = FVX_ERR_NOT_CONVERGED;
+	const double wall = now();
+	for(int i = 0; i < ninst; i++) {
+		fvx_solve_stats& st = stats[i];
+		st.walltime = single ? wall : t_end[i];
+		st.steps = is[i].step;
+		st.final_abs = is[i].resi;
+		st.first_norm = is[i].initres;
+		st.final_rel = is[i].resi/is[i].initres;
+		st.converged = (st.status == 0 && st.final_rel <= c.tol) ? 1 : 0;
+		if(st.status == 0 && !st.converged) st.status = FVX_ERR_NOT_CONVERGED;
+	}
+}
+
+void steady_solve(fvx_disc_s& d, const fvx_solver_cfg& c, double* u, fvx_solve_stats& st,
+                  hipStream_t s)
+{
+	std::vector<fvx_solve_stats> v;
+	steady_core(d, c, u, 1, nullptr, v, s);
+	st = v[0];
 	if(st.status == FVX_ERR_NONFINITE)
 		throw Error(FVX_ERR_NONFINITE, "non-finite residual in the pseudo-time loop");
 	if(st.status == FVX_ERR_NOT_CONVERGED)

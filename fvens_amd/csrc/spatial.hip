@@ -17,8 +17,9 @@ namespace fvx {
 
 #define FVX_DECL_FLUX(ID) void launch_flux_##ID(int mode, bool visc, int ncell, const double* u, \
 	const double* rgrad, const double* grad, const int* nbr, const int* face, const FaceGeom* fg, \
-	const double* centre, const double* area, const int* bcid, const GasParams& gp, \
-	const BCTable& bt, double* res, double* dt, int getdt, hipStream_t s);
+	const double* centre, const double* area, const int* bcid, const GasParams* gpa, \
+	const BCTable* bta, double* res, double* dt, int getdt, const BatchArg& ba, int nact, \
+	int block, int xcd, hipStream_t s);
 FVX_DECL_FLUX(0) FVX_DECL_FLUX(1) FVX_DECL_FLUX(2) FVX_DECL_FLUX(3) FVX_DECL_FLUX(4)
 FVX_DECL_FLUX(5) FVX_DECL_FLUX(6)
 #undef FVX_DECL_FLUX
@@ -29,16 +30,25 @@ FVX_DECL_FLUX(5) FVX_DECL_FLUX(6)
 /// Two lanes per cell, each owning two of the cell's local faces; the pair combines its
 /// partial sums, bounds and limiter values with one DPP exchange.
 template <int LIM, bool CONS>
-__global__ __launch_bounds__(BLOCK)
+__global__ __launch_bounds__(MAX_BLOCK)
 void k_gradients(int ncell, const double* __restrict__ u, const int* __restrict__ nbr,
                  const int* __restrict__ face, const FaceGeom* __restrict__ fgeom,
                  const double* __restrict__ centre, const double* __restrict__ gcoef,
                  const double* __restrict__ eps2, const int* __restrict__ bcid,
-                 const GasParams gp, const BCTable bt, double* __restrict__ gout,
-                 double* __restrict__ rout)
+                 const GasParams* __restrict__ gpa, const BCTable* __restrict__ bta,
+                 const BatchArg ba, double* __restrict__ gout, double* __restrict__ rout,
+                 const int xcd)
 {
+	// instance: state and gradients are instance-major, 4 and 8 values per cell; gpa holds
+	// the gas parameters of each instance (one entry for a single solve)
+	const long I = ba.list ? ba.list[blockIdx.y] : 0;
+	const GasParams& gp = gpa[I];
+	const BCTable& bt = *bta;
+	u += I*4L*ncell;
+	if(gout) gout += I*8L*ncell;
+	if(rout) rout += I*8L*ncell;
 	const int p = threadIdx.x & 1;
-	const int cell = xcd_block()*(BLOCK/2) + (threadIdx.x >> 1);
+	const int cell = xcd_block(xcd)*(int)(blockDim.x >> 1) + (threadIdx.x >> 1);
 	const bool valid = cell < ncell;
 	const int cc = valid ? cell : 0;
 	double ui[4], wi[4];
@@ -180,15 +190,17 @@ Profiler::~Profiler()
 }
 
 void launch_gradients(const fvx_disc_s& d, const double* u, double* grad, double* rgrad,
-                      int conserved, hipStream_t s)
+                      int conserved, hipStream_t s, const BatchArg& ba, int nact)
 {
 	const fvx_mesh_s& m = *d.m;
-	const int nblk = (m.nelem + BLOCK/2 - 1)/(BLOCK/2);
+	const int block = d.tune.grad_block, cpb = block/2;
+	const int nblk = (m.nelem + cpb - 1)/cpb;
 	const int lim = conserved ? 0 : (d.nc.reconstruction == FVX_RECON_VENKATAKRISHNAN ? 1 :
 	                                 d.nc.reconstruction == FVX_RECON_BARTHJESPERSEN ? 2 : 0);
-#define FVX_GL(L, C) hipLaunchKernelGGL((k_gradients<L, C>), dim3(nblk), dim3(BLOCK), 0, s, \
-		m.nelem, u, m.d_nbr.p, m.d_face.p, m.d_fgeom.p, m.d_centre.p, d.d_gcoef.p, d.d_eps2.p, \
-		d.d_bcid.p, d.gas, d.bct, grad, rgrad)
+#define FVX_GL(L, C) hipLaunchKernelGGL((k_gradients<L, C>), dim3(nblk, nact), dim3(block), 0, \
+		s, m.nelem, u, m.d_nbr.p, m.d_face.p, m.d_fgeom.p, m.d_centre.p, d.d_gcoef.p, \
+		d.d_eps2.p, d.d_bcid.p, ba.gp ? ba.gp : d.d_gas.p, d.d_bct.p, ba, grad, rgrad, \
+		d.tune.grad_xcd)
 	d.prof.start(PROF_GRADIENTS, s);
 	if(conserved) FVX_GL(0, true);
 	else if(lim == 0) FVX_GL(0, false);
@@ -200,9 +212,12 @@ void launch_gradients(const fvx_disc_s& d, const double* u, double* grad, double
 }
 
 void launch_residual(const fvx_disc_s& d, const double* u, double* r, int gettimesteps,
-                     double* dt, hipStream_t s)
+                     double* dt, hipStream_t s, const BatchArg& ba, int nact, double* grad,
+                     double* rgrad)
 {
 	const fvx_mesh_s& m = *d.m;
+	if(!grad) grad = d.d_grad.p;
+	if(!rgrad) rgrad = d.d_rgrad.p;
 	const bool order2 = d.nc.order2 && d.nc.gradientscheme != FVX_GRAD_NONE;
 	const bool visc = d.pc.viscous != 0;
 	const int rec = d.nc.reconstruction;
@@ -212,19 +227,20 @@ void launch_residual(const fvx_disc_s& d, const double* u, double* r, int gettim
 	if(order2) {
 		const bool limited = rec == FVX_RECON_VENKATAKRISHNAN || rec == FVX_RECON_BARTHJESPERSEN;
 		if(limited) {
-			launch_gradients(d, u, visc ? d.d_grad.p : nullptr, d.d_rgrad.p, 0, s);
-			rg = d.d_rgrad.p; ug = d.d_grad.p; mode = 1;
+			launch_gradients(d, u, visc ? grad : nullptr, rgrad, 0, s, ba, nact);
+			rg = rgrad; ug = grad; mode = 1;
 		} else {
-			launch_gradients(d, u, d.d_grad.p, nullptr, 0, s);
-			rg = d.d_grad.p; ug = d.d_grad.p;
+			launch_gradients(d, u, grad, nullptr, 0, s, ba, nact);
+			rg = grad; ug = grad;
 			mode = rec == FVX_RECON_VANALBADA ? 2 : 1;
 		}
 	}
 	d.prof.start(PROF_FLUX, s);
 	switch(d.nc.conv_numflux) {
 #define FVX_C(ID) case ID: launch_flux_##ID(mode, visc, m.nelem, u, rg, ug, m.d_nbr.p, \
-	m.d_face.p, m.d_fgeom.p, m.d_centre.p, m.d_area.p, d.d_bcid.p, d.gas, d.bct, r, dt, \
-	gettimesteps, s); break;
+	m.d_face.p, m.d_fgeom.p, m.d_centre.p, m.d_area.p, d.d_bcid.p, \
+	ba.gp ? ba.gp : d.d_gas.p, d.d_bct.p, r, dt, gettimesteps, ba, nact, d.tune.flux_block, \
+	d.tune.flux_xcd, s); break;
 	FVX_C(0) FVX_C(1) FVX_C(2) FVX_C(3) FVX_C(4) FVX_C(5) FVX_C(6)
 #undef FVX_C
 	default: throw Error(FVX_ERR_INVALID, "unknown flux");
@@ -289,12 +305,13 @@ void k_functionals(int ncell, int nbface, const double* __restrict__ u,
 }
 
 void launch_functionals(const fvx_disc_s& d, const double* u, const double* cgrad, int walltag,
-                        double* dev_out, hipStream_t s)
+                        double* dev_out, hipStream_t s, const GasParams* gp, double aoa)
 {
 	const fvx_mesh_s& m = *d.m;
+	const double a = gp ? aoa : d.pc.aoa;
 	hipLaunchKernelGGL(k_functionals, dim3(FN_BLOCKS), dim3(BLOCK), 0, s, m.nelem, m.nbface, u,
-	                   cgrad, m.d_bhost.p, m.d_btag.p, m.d_fgeom.p, m.d_area.p, walltag, d.gas,
-	                   cos(d.pc.aoa), sin(d.pc.aoa), dev_out);
+	                   cgrad, m.d_bhost.p, m.d_btag.p, m.d_fgeom.p, m.d_area.p, walltag,
+	                   gp ? *gp : d.gas, cos(a), sin(a), dev_out);
 	FVX_HIP(hipGetLastError());
 }
 

@@ -15,6 +15,7 @@ namespace fvx {
 
 constexpr int CELLS_PER_BLOCK = 64;
 constexpr int BLOCK = 256;
+constexpr int MAX_BLOCK = 512;   ///< largest workgroup the tunable cell kernels are built for
 
 // DPP quad_perm selectors: lane^1 = [1,0,3,2], lane^2 = [2,3,0,1]
 constexpr int DPP_XOR1 = 0xB1;
@@ -54,10 +55,11 @@ __device__ __forceinline__ double pair_min(double v) { return fmin(v, dpp_d<DPP_
 
 /// Logical workgroup index such that each XCD (workgroups are dealt round-robin over the 8
 /// XCDs) processes one contiguous range of cells: neighbours gathered by one workgroup are
-/// then mostly resident in the same XCD's L2.
-__device__ __forceinline__ int xcd_block()
+/// then mostly resident in the same XCD's L2. With on == false, the hardware order.
+__device__ __forceinline__ int xcd_block(bool on = true)
 {
 	const int b = blockIdx.x, nb = gridDim.x;
+	if(!on) return b;
 	const int q = nb >> 3, r = nb & 7, x = b & 7, l = b >> 3;
 	return x*q + (x < r ? x : r) + l;
 }

@@ -223,6 +223,37 @@ class FlowFV:
 
     KERNELS = {"gradients": 0, "flux": 1, "jacobian": 2, "pc": 3}
 
+    def set_launch(self, kernel, block, xcd=True):
+        check(lib().fvx_set_launch(self.h, self.KERNELS[kernel], int(block), int(xcd)))
+
+    def get_launch(self, kernel):
+        b, x = C.c_int(), C.c_int()
+        check(lib().fvx_get_launch(self.h, self.KERNELS[kernel], C.byref(b), C.byref(x)))
+        return b.value, bool(x.value)
+
+    def autotune(self, u, reps=20):
+        """Times every launch shape of the cell kernels at state u, keeps the fastest and
+        returns the timing table [(kernel, block, xcd, ms)]."""
+        rows = (TuneRow * 64)()
+        nr = C.c_int()
+        check(lib().fvx_autotune(self.h, _dptr(u, 4 * self.mesh.nelem), int(reps), rows, 64,
+                                 C.byref(nr), _stream()))
+        names = {v: k for k, v in self.KERNELS.items()}
+        return [(names[r.kernel], r.block, bool(r.xcd), r.ms) for r in rows[:nr.value]]
+
+    def autotune_pc(self, u, cfl=1000.0, restart=30, rtol=0.1, precision="f64"):
+        """Times preconditioner types and sweep counts on the pseudo-time system at u; returns
+        ((best type, best sweeps), [(type, sweeps, iterations, ms)]), ms < 0 where the solve
+        did not reach rtol."""
+        rows = (TuneRow * 64)()
+        nr, bp, bs = C.c_int(), C.c_int(), C.c_int()
+        check(lib().fvx_autotune_pc(self.h, _dptr(u, 4 * self.mesh.nelem), C.c_double(cfl),
+                                    int(restart), C.c_double(rtol), PREC[precision], rows, 64,
+                                    C.byref(nr), C.byref(bp), C.byref(bs), _stream()))
+        names = {v: k for k, v in PC.items()}
+        table = [(names[r.kernel], r.block, r.xcd, r.ms) for r in rows[:nr.value]]
+        return (names[bp.value], bs.value), table
+
     def profile_enable(self, on=True):
         check(lib().fvx_profile_enable(self.h, int(on)))
 
@@ -272,6 +303,10 @@ class FlowFV:
 
 
 PREC = {"f64": 0, "f32": 1}
+
+
+class TuneRow(C.Structure):
+    _fields_ = [("kernel", C.c_int), ("block", C.c_int), ("xcd", C.c_int), ("ms", C.c_double)]
 PC = {"none": 0, "jacobi": 1, "sgs": 2, "ilu0": 3}
 
 
@@ -379,3 +414,53 @@ def linear_solve(A: Matrix, pc: Preconditioner, b, x, restart=30, rtol=1e-1):
                                  C.c_double(rtol),
                                  C.byref(it), _stream()))
     return it.value
+
+
+class Batch:
+    """Operating points (Mach, angle of attack) solved together on one discretisation; state
+    tensors are [ninst, ncell, 4]."""
+
+    def __init__(self, disc: FlowFV, mach, aoa_deg):
+        mach = np.ascontiguousarray(mach, np.float64)
+        aoa = np.ascontiguousarray(np.deg2rad(aoa_deg), np.float64)
+        if mach.shape != aoa.shape or mach.ndim != 1:
+            raise ValueError("mach and aoa_deg must be 1-D arrays of equal length")
+        self.disc, self.ninst = disc, len(mach)
+        h = C.c_void_p()
+        check(lib().fvx_batch_create(disc.h, self.ninst, _ptr(mach, C.c_double),
+                                     _ptr(aoa, C.c_double), C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.fvx_batch_destroy(self.h)
+            self.h = None
+
+    def freestream(self) -> np.ndarray:
+        out = np.zeros((self.ninst, 4))
+        check(lib().fvx_batch_freestream(self.h, _ptr(out, C.c_double)))
+        return out
+
+    def compute_residual(self, u, r, gettimesteps=False, dt=None):
+        n = self.ninst * self.disc.mesh.nelem
+        if gettimesteps and dt is None:
+            raise ValueError("time steps requested without an output tensor")
+        check(lib().fvx_batch_compute_residual(self.h, _dptr(u, 4 * n), _dptr(r, 4 * n),
+                                               int(gettimesteps),
+                                               _dptr(dt, n) if dt is not None else C.c_void_p(0),
+                                               _stream()))
+
+    def steady_solve(self, u, **kw):
+        """Solves every instance in place; returns one statistics dict per instance."""
+        cfg = solver_cfg(**kw)
+        st = (SolveStats * self.ninst)()
+        check(lib().fvx_batch_steady_solve(self.h, C.byref(cfg),
+                                           _dptr(u, 4 * self.ninst * self.disc.mesh.nelem), st,
+                                           _stream()))
+        return [{f: getattr(s, f) for f, _ in SolveStats._fields_} for s in st]
+
+    def functionals(self, u, wall_tag=2):
+        out = np.zeros((self.ninst, 4))
+        check(lib().fvx_batch_functionals(self.h, _dptr(u, 4 * self.ninst * self.disc.mesh.nelem),
+                                          int(wall_tag), _ptr(out, C.c_double), _stream()))
+        return [dict(CL=o[0], CDp=o[1], CDsf=o[2], entropy=o[3]) for o in out]

@@ -220,6 +220,57 @@ typedef struct {
 fvx_status fvx_steady_solve(fvx_disc d, const fvx_solver_cfg* cfg, double* u,
                             fvx_solve_stats* stats, void* stream);
 
+/* ---- batched operating points ----
+ * No reference counterpart: the reference runs one operating point per process. A batch holds
+ * ninst operating points (Mach number, angle of attack in radians) on one discretisation;
+ * every other setting is the discretisation's. Batched state vectors are instance-major device
+ * arrays [ninst][ncell][4]. Each launch covers all active instances, while every instance keeps
+ * its own Krylov iterations, CFL ramp, convergence test and retirement, so instance i evolves
+ * exactly as a single solve of the same operating point would. */
+typedef struct fvx_batch_s* fvx_batch;
+fvx_status fvx_batch_create(fvx_disc d, int ninst, const double* mach, const double* aoa,
+                            fvx_batch* out);
+fvx_status fvx_batch_destroy(fvx_batch b);
+/* Free-stream states, host out[ninst][4]. */
+fvx_status fvx_batch_freestream(fvx_batch b, double* out);
+/* fvx_compute_residual for every instance (u, r: [ninst][ncell][4]; dt: [ninst][ncell]). */
+fvx_status fvx_batch_compute_residual(fvx_batch b, const double* u, double* r, int gettimesteps,
+                                      double* dt, void* stream);
+/* fvx_steady_solve for every instance; stats[ninst] receives each instance's outcome in its
+ * status field (FVX_OK, FVX_ERR_NONFINITE or FVX_ERR_NOT_CONVERGED); a failing instance does
+ * not stop the others, and the call itself returns FVX_OK unless an argument or the device
+ * fails. */
+fvx_status fvx_batch_steady_solve(fvx_batch b, const fvx_solver_cfg* cfg, double* u,
+                                  fvx_solve_stats* stats, void* stream);
+/* fvx_functionals for every instance, host out[ninst][4]. */
+fvx_status fvx_batch_functionals(fvx_batch b, const double* u, int wall_tag, double* out,
+                                 void* stream);
+
+/* ---- autotuning (no reference counterpart: OpenMP loops have no launch shape) ---- */
+
+/* One timing: kernel id (FVX_KERNEL_*), threads per workgroup, XCD-aware block order
+ * (0/1) and milliseconds per launch. For fvx_autotune_pc: kernel = FVX_PC_* type, block =
+ * sweeps, xcd = Krylov iterations, ms = set-up + solve time (negative: did not reach rtol). */
+typedef struct {
+	int kernel, block, xcd;
+	double ms;
+} fvx_tune_row;
+
+/* Times every launch shape of the gradient, face-flux and Jacobian kernels at state u and
+ * keeps the fastest in d (results are bitwise identical for every shape). rows[cap]. */
+fvx_status fvx_autotune(fvx_disc d, const double* u, int reps, fvx_tune_row* rows, int cap,
+                        int* nrows, void* stream);
+/* Sets / reads the launch shape of one kernel (block 64, 128, 256 or 512; 256 at most for the
+ * Jacobian). */
+fvx_status fvx_set_launch(fvx_disc d, int kernel, int block, int xcd);
+fvx_status fvx_get_launch(fvx_disc d, int kernel, int* block, int* xcd);
+/* Times preconditioner types and sweep counts on the pseudo-time linear system at u and the
+ * given CFL number (set-up + one FGMRES(restart) solve to rtol) and returns the fastest that
+ * reaches rtol. */
+fvx_status fvx_autotune_pc(fvx_disc d, const double* u, double cfl, int restart, double rtol,
+                           int precision, fvx_tune_row* rows, int cap, int* nrows, int* best_pc,
+                           int* best_sweeps, void* stream);
+
 /* ---- timing: the TimingData role of the reference (src/ode/aodesolver.hpp:46-67) ---- */
 
 /* Kernel ids for fvx_profile_read. */
