@@ -489,4 +489,154 @@ fvx_status fvx_functionals(fvx_disc d, const double* u, int wall_tag, double* ou
 	});
 }
 
+
+// ---- learned warm start ----
+
+int fvx_mpnn_nparams(void) { return fvx::mpnn::NPARAM; }
+
+fvx_status fvx_mpnn_create(fvx_mesh m, int wall_tag, int max_graphs, fvx_mpnn* out)
+{
+	return guarded([&] {
+		require(m && out, "null argument");
+		require(max_graphs >= 1 && (long long)max_graphs*m->nelem < (1ll << 31) - 16,
+		        "max_graphs must be at least 1 and keep graphs x cells below 2^31");
+		std::unique_ptr<fvx_mpnn_s> n(new fvx_mpnn_s);
+		n->net.init(*m, wall_tag, max_graphs);
+		*out = n.release();
+	});
+}
+
+fvx_status fvx_mpnn_destroy(fvx_mpnn n)
+{
+	return guarded([&] { delete n; });
+}
+
+fvx_status fvx_mpnn_features(fvx_mpnn n, float* stat, float* edge, int* nbr, int* rev,
+                             float* invdeg)
+{
+	return guarded([&] {
+		require(n != nullptr, "null handle");
+		const auto& g = n->net.host;
+		if(stat) std::memcpy(stat, g.stat.data(), g.stat.size()*sizeof(float));
+		if(edge) std::memcpy(edge, g.edge.data(), g.edge.size()*sizeof(float));
+		if(nbr) std::memcpy(nbr, g.nbr.data(), g.nbr.size()*sizeof(int));
+		if(rev) std::memcpy(rev, g.rev.data(), g.rev.size()*sizeof(int));
+		if(invdeg) std::memcpy(invdeg, g.invdeg.data(), g.invdeg.size()*sizeof(float));
+	});
+}
+
+fvx_status fvx_mpnn_set_params(fvx_mpnn n, const float* w)
+{
+	return guarded([&] {
+		require(n && w, "null argument");
+		fvx::Mpnn& net = n->net;
+		const size_t bytes = fvx::mpnn::NPARAM*sizeof(float);
+		FVX_HIP(hipDeviceSynchronize());
+		FVX_HIP(hipMemcpy(net.w.p, w, bytes, hipMemcpyHostToDevice));
+		FVX_HIP(hipMemset(net.am.p, 0, bytes));
+		FVX_HIP(hipMemset(net.av.p, 0, bytes));
+		net.step = 0;
+		net.dirty = true;
+	});
+}
+
+fvx_status fvx_mpnn_get_params(fvx_mpnn n, float* w)
+{
+	return guarded([&] {
+		require(n && w, "null argument");
+		FVX_HIP(hipDeviceSynchronize());
+		FVX_HIP(hipMemcpy(w, n->net.w.p, fvx::mpnn::NPARAM*sizeof(float), hipMemcpyDeviceToHost));
+	});
+}
+
+fvx_status fvx_mpnn_set_points(fvx_mpnn n, int ngraph, const double* mach, const double* aoa,
+                               double gamma, void* stream)
+{
+	return guarded([&] {
+		require(n && mach && aoa, "null argument");
+		require(gamma > 1, "gamma must exceed 1");
+		n->net.set_points(ngraph, mach, aoa, gamma, static_cast<hipStream_t>(stream));
+	});
+}
+
+namespace {
+void require_points(const fvx_mpnn n, int ngraph)
+{
+	require(ngraph >= 1 && ngraph <= n->net.npoints,
+	        "ngraph must be between 1 and the number of points set");
+}
+}
+
+fvx_status fvx_mpnn_predict(fvx_mpnn n, int ngraph, float* y, void* stream)
+{
+	return guarded([&] {
+		require(n && y, "null argument");
+		require_points(n, ngraph);
+		n->net.predict(ngraph, y, static_cast<hipStream_t>(stream));
+	});
+}
+
+fvx_status fvx_mpnn_warm_state(fvx_mpnn n, int ngraph, double floor, double* u, void* stream)
+{
+	return guarded([&] {
+		require(n && u, "null argument");
+		require_points(n, ngraph);
+		require(floor > 0 && floor <= 1, "floor must be in (0, 1]");
+		n->net.warm(ngraph, floor, u, static_cast<hipStream_t>(stream));
+	});
+}
+
+fvx_status fvx_mpnn_targets(fvx_mpnn n, int ngraph, const double* u, float* t, void* stream)
+{
+	return guarded([&] {
+		require(n && u && t, "null argument");
+		require_points(n, ngraph);
+		n->net.targets(ngraph, u, t, static_cast<hipStream_t>(stream));
+	});
+}
+
+namespace {
+void mpnn_grad(fvx_mpnn n, int ngraph, const float* targets, const int* ids, hipStream_t s)
+{
+	require(n && targets && ids, "null argument");
+	require_points(n, ngraph);
+	n->net.gradient(ngraph, targets, ids, s);
+}
+
+/// The device holds the summed squared error; the loss is its mean over cells and outputs.
+void read_loss(fvx_mpnn n, int ngraph, float* loss, hipStream_t s)
+{
+	if(!loss) return;
+	float sum = 0;
+	FVX_HIP(hipMemcpyAsync(&sum, n->net.loss.p, sizeof(float), hipMemcpyDeviceToHost, s));
+	FVX_HIP(hipStreamSynchronize(s));
+	*loss = (float)(sum/((double)ngraph*n->net.N*fvx::mpnn::FO));
+}
+}
+
+fvx_status fvx_mpnn_gradient(fvx_mpnn n, int ngraph, const float* targets, const int* ids,
+                             float* grad, float* loss, void* stream)
+{
+	return guarded([&] {
+		hipStream_t s = static_cast<hipStream_t>(stream);
+		mpnn_grad(n, ngraph, targets, ids, s);
+		if(grad)
+			FVX_HIP(hipMemcpyAsync(grad, n->net.grad.p, fvx::mpnn::NPARAM*sizeof(float),
+			                       hipMemcpyDeviceToDevice, s));
+		read_loss(n, ngraph, loss, s);
+	});
+}
+
+fvx_status fvx_mpnn_train_step(fvx_mpnn n, int ngraph, const float* targets, const int* ids,
+                               double lr, float* loss, void* stream)
+{
+	return guarded([&] {
+		require(lr > 0 && std::isfinite(lr), "learning rate must be positive");
+		hipStream_t s = static_cast<hipStream_t>(stream);
+		mpnn_grad(n, ngraph, targets, ids, s);
+		n->net.adam_step(lr, s);
+		read_loss(n, ngraph, loss, s);
+	});
+}
+
 }

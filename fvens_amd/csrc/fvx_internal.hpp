@@ -272,6 +272,64 @@ void autotune_launch(fvx_disc_s& d, const double* u, int reps, std::vector<fvx_t
 void autotune_pc(fvx_disc_s& d, const double* u, double cfl, int restart, double rtol,
                  int precision, std::vector<fvx_tune_row>& rows, int& best_pc, int& best_sweeps,
                  hipStream_t s);
+
+// learned warm start (mpnn.hip)
+namespace mpnn {
+constexpr int W = 64, NL = 4, FIN = 8, FE = 4, FO = 4, NSTAT = 6;
+// flat fp32 parameter layout; matrices are [in][out] row-major
+constexpr int OFF_WENC = 0, OFF_BENC = 512, OFF_L0 = 576, LSZ = 16768;
+constexpr int L_WA = 0, L_WB = 4096, L_WE = 8192, L_BM = 8448, L_WU = 8512, L_WV = 12608,
+              L_BU = 16704;
+constexpr int OFF_WDEC = OFF_L0 + NL*LSZ, OFF_BDEC = OFF_WDEC + 256, NPARAM = OFF_BDEC + 4;
+static_assert(NPARAM == 67908, "parameter layout");
+// packed MFMA B operands per layer: P|Q forward, update forward, update backward, P|Q backward
+enum { M_PQ = 0, M_UPD = 1, M_BUPD = 2, M_BPQ = 3 };
+constexpr int FRAG_MAT = 16*64*8;      // bf16 values per packed matrix (16 fragments)
 }
+
+/// Graph arrays of the learned warm start built on the host from a mesh.
+struct MpnnGraphHost {
+	int N = 0;
+	std::vector<int> nbr, rev;             // [N][4] interior neighbour / reverse slot, -1 unused
+	std::vector<float> invdeg, stat, edge; // [N], [N][6], [N][4][4]
+};
+void mpnn_build_graph(const fvx_mesh_s& m, int wall_tag, MpnnGraphHost& gh);
+
+struct Mpnn {
+	MpnnGraphHost host;
+	int N = 0, cap = 0;
+	long step = 0;
+	bool dirty = true;
+	double gamma = 1.4;
+	DBuf<int> d_nbr, d_rev, ids;
+	DBuf<float> d_invdeg, d_stat, d_edge;
+	DBuf<float> w, wq, am, av, grad, loss;
+	DBuf<unsigned short> frag;
+	DBuf<float> h[mpnn::NL + 1], pq[mpnn::NL];
+	DBuf<unsigned short> ab[mpnn::NL];
+	DBuf<unsigned long long> umask[mpnn::NL];
+	DBuf<float> g0, g1, gpart, da, y, part, gfeat;
+	DBuf<unsigned short> dpre, dpq;
+	DBuf<double> pp;
+	std::vector<float> hgf;
+	std::vector<double> hpp;
+	std::vector<int> hids;
+
+	void init(const fvx_mesh_s& m, int wall_tag, int max_graphs);
+	int npoints = 0;                       // graphs described by the last set_points
+	void set_points(int G, const double* mach, const double* aoa, double gamma, hipStream_t s);
+	void pack(hipStream_t s);
+	void forward(int G, bool train, hipStream_t s);
+	void predict(int G, float* yout, hipStream_t s);
+	void gradient(int G, const float* targets, const int* sample_ids, hipStream_t s);
+	void adam_step(double lr, hipStream_t s);
+	void targets(int G, const double* u, float* t, hipStream_t s);
+	void warm(int G, double floor, double* u, hipStream_t s);
+};
+}
+
+struct fvx_mpnn_s {
+	fvx::Mpnn net;
+};
 
 #endif

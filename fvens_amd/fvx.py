@@ -464,3 +464,104 @@ class Batch:
         check(lib().fvx_batch_functionals(self.h, _dptr(u, 4 * self.ninst * self.disc.mesh.nelem),
                                           int(wall_tag), _ptr(out, C.c_double), _stream()))
         return [dict(CL=o[0], CDp=o[1], CDsf=o[2], entropy=o[3]) for o in out]
+
+
+def _fptr(t, numel, dtype="float32"):
+    """Device pointer of a contiguous device tensor of `numel` elements of `dtype`."""
+    import torch
+    want = getattr(torch, dtype)
+    if not (t.is_cuda and t.is_contiguous()) or t.dtype != want or t.numel() != numel:
+        raise ValueError(f"expected a contiguous device tensor of {numel} {dtype} elements")
+    return C.c_void_p(t.data_ptr())
+
+
+class MPNN:
+    """Learned warm start on one mesh: a 4-layer message-passing network (width 64, bf16
+    weights, fp32 accumulation) trained and evaluated by the library's device kernels.
+
+    Per-graph tensors are [ngraph, ncell, 4]; angles of attack are in degrees here."""
+
+    def __init__(self, mesh: Mesh, wall_tag: int, max_graphs: int, gamma: float = 1.4):
+        h = C.c_void_p()
+        check(lib().fvx_mpnn_create(mesh.h, int(wall_tag), int(max_graphs), C.byref(h)))
+        self.h, self.mesh, self.max_graphs, self.gamma = h, mesh, int(max_graphs), gamma
+        self.N = mesh.nelem
+        self.npoints = 0
+
+    def __del__(self):
+        if getattr(self, "h", None) is not None and _lib is not None:
+            _lib.fvx_mpnn_destroy(self.h)
+            self.h = None
+
+    @staticmethod
+    def nparams() -> int:
+        return int(lib().fvx_mpnn_nparams())
+
+    def features(self):
+        N = self.N
+        out = dict(stat=np.zeros((N, 6), np.float32), edge=np.zeros((N, 4, 4), np.float32),
+                   nbr=np.zeros((N, 4), np.int32), rev=np.zeros((N, 4), np.int32),
+                   invdeg=np.zeros(N, np.float32))
+        check(lib().fvx_mpnn_features(self.h, _ptr(out["stat"], C.c_float),
+                                      _ptr(out["edge"], C.c_float), _ptr(out["nbr"], C.c_int),
+                                      _ptr(out["rev"], C.c_int), _ptr(out["invdeg"], C.c_float)))
+        return out
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, np.float32)
+        if w.shape != (self.nparams(),):
+            raise ValueError("parameter vector has the wrong length")
+        check(lib().fvx_mpnn_set_params(self.h, _ptr(w, C.c_float)))
+
+    def get_params(self) -> np.ndarray:
+        w = np.zeros(self.nparams(), np.float32)
+        check(lib().fvx_mpnn_get_params(self.h, _ptr(w, C.c_float)))
+        return w
+
+    def set_points(self, mach, aoa_deg):
+        mach = np.ascontiguousarray(mach, np.float64)
+        aoa = np.ascontiguousarray(np.deg2rad(aoa_deg), np.float64)
+        if mach.shape != aoa.shape or mach.ndim != 1:
+            raise ValueError("mach and aoa_deg must be 1-D arrays of equal length")
+        check(lib().fvx_mpnn_set_points(self.h, len(mach), _ptr(mach, C.c_double),
+                                        _ptr(aoa, C.c_double), C.c_double(self.gamma), _stream()))
+        self.npoints = len(mach)
+
+    def predict(self, y, ngraph=None):
+        G = ngraph or self.npoints
+        check(lib().fvx_mpnn_predict(self.h, G, _fptr(y, G * self.N * 4), _stream()))
+
+    def warm_state(self, u, ngraph=None, floor=0.05):
+        G = ngraph or self.npoints
+        check(lib().fvx_mpnn_warm_state(self.h, G, C.c_double(floor),
+                                        _fptr(u, G * self.N * 4, "float64"), _stream()))
+
+    def targets(self, u, t, ngraph=None):
+        G = ngraph or self.npoints
+        check(lib().fvx_mpnn_targets(self.h, G, _fptr(u, G * self.N * 4, "float64"),
+                                     _fptr(t, G * self.N * 4), _stream()))
+
+    def _ids(self, ids, targets):
+        ids = np.ascontiguousarray(ids, np.int32)
+        nsamp = targets.numel() // (self.N * 4)
+        if targets.numel() != nsamp * self.N * 4 or ids.ndim != 1 or len(ids) > self.npoints:
+            raise ValueError("targets must be [nsamples, ncell, 4] and ids at most one per point")
+        if len(ids) and (ids.min() < 0 or ids.max() >= nsamp):
+            raise ValueError("sample index out of range")
+        return ids, nsamp
+
+    def gradient(self, targets, ids, grad=None):
+        ids, nsamp = self._ids(ids, targets)
+        loss = C.c_float()
+        gp = _fptr(grad, self.nparams()) if grad is not None else C.c_void_p(0)
+        check(lib().fvx_mpnn_gradient(self.h, len(ids), _fptr(targets, nsamp * self.N * 4),
+                                      _ptr(ids, C.c_int), gp, C.byref(loss), _stream()))
+        return float(loss.value)
+
+    def train_step(self, targets, ids, lr, want_loss=True):
+        ids, nsamp = self._ids(ids, targets)
+        loss = C.c_float()
+        check(lib().fvx_mpnn_train_step(self.h, len(ids), _fptr(targets, nsamp * self.N * 4),
+                                        _ptr(ids, C.c_int), C.c_double(lr),
+                                        C.byref(loss) if want_loss else C.c_void_p(0), _stream()))
+        return float(loss.value) if want_loss else None

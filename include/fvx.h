@@ -271,6 +271,51 @@ fvx_status fvx_autotune_pc(fvx_disc d, const double* u, double cfl, int restart,
                            int precision, fvx_tune_row* rows, int cap, int* nrows, int* best_pc,
                            int* best_sweeps, void* stream);
 
+/* ---- learned warm start (no reference counterpart: the reference starts a solve from the
+ * freestream or from a first-order starter solve, src/utilities/casesolvers.cpp:225-386) ----
+ *
+ * A 4-layer message-passing network over the cell graph of a mesh (width 64, bf16 weights,
+ * fp32 accumulation) maps (Mach number, angle of attack, cell geometry) to a predicted state;
+ * training and inference run on the device. Network outputs per cell are the normalised
+ * deviations (rho - 1, vx - cos(aoa), vy - sin(aoa), p/p_inf - 1) from the reference's
+ * non-dimensional freestream (src/physics/aphysics.cpp:44-58). Graphs of one call share the mesh;
+ * per-graph arrays are graph-major [ngraph][ncell][...]. */
+typedef struct fvx_mpnn_s* fvx_mpnn;
+/* Number of fp32 parameters (flat layout: Wenc[8][64], benc[64], then per layer Wa, Wb [64][64],
+ * We[4][64], bm[64], Wu, Wv [64][64], bu[64], then Wdec[64][4], bdec[4]). */
+int fvx_mpnn_nparams(void);
+/* Network on mesh m; wall_tag marks the wall faces (wall distance feature); max_graphs bounds
+ * the graphs per call. Parameters start at zero: call fvx_mpnn_set_params. */
+fvx_status fvx_mpnn_create(fvx_mesh m, int wall_tag, int max_graphs, fvx_mpnn* out);
+fvx_status fvx_mpnn_destroy(fvx_mpnn n);
+/* Graph arrays (host outs, any may be NULL): static node features [ncell][6], edge features
+ * [ncell][4][4], interior neighbour [ncell][4] and reverse slot [ncell][4] (-1 where unused),
+ * 1/degree [ncell]. */
+fvx_status fvx_mpnn_features(fvx_mpnn n, float* stat, float* edge, int* nbr, int* rev,
+                             float* invdeg);
+/* Master weights from / to host arrays [nparams]; setting them also resets the optimiser. */
+fvx_status fvx_mpnn_set_params(fvx_mpnn n, const float* w);
+fvx_status fvx_mpnn_get_params(fvx_mpnn n, float* w);
+/* Operating points of the next calls: host mach[ngraph], aoa[ngraph] (radians). */
+fvx_status fvx_mpnn_set_points(fvx_mpnn n, int ngraph, const double* mach, const double* aoa,
+                               double gamma, void* stream);
+/* Network outputs for the first ngraph points, device y[ngraph][ncell][4]. */
+fvx_status fvx_mpnn_predict(fvx_mpnn n, int ngraph, float* y, void* stream);
+/* Predicted conserved state, device u[ngraph][ncell][4]; density and pressure are floored at
+ * `floor` times their freestream values. */
+fvx_status fvx_mpnn_warm_state(fvx_mpnn n, int ngraph, double floor, double* u, void* stream);
+/* Training targets (the outputs a perfect network would give) of the conserved fields
+ * u[ngraph][ncell][4], device t[ngraph][ncell][4]. */
+fvx_status fvx_mpnn_targets(fvx_mpnn n, int ngraph, const double* u, float* t, void* stream);
+/* Mean-squared loss over the first ngraph points against targets of a data set (device
+ * [nsamples][ncell][4]; graph g uses sample ids[g], host array) and its gradient (device
+ * grad[nparams], may be NULL); *loss (host, may be NULL) waits for the result. */
+fvx_status fvx_mpnn_gradient(fvx_mpnn n, int ngraph, const float* targets, const int* ids,
+                             float* grad, float* loss, void* stream);
+/* fvx_mpnn_gradient followed by one Adam step (beta 0.9 / 0.999, eps 1e-8) at rate lr. */
+fvx_status fvx_mpnn_train_step(fvx_mpnn n, int ngraph, const float* targets, const int* ids,
+                               double lr, float* loss, void* stream);
+
 /* ---- timing: the TimingData role of the reference (src/ode/aodesolver.hpp:46-67) ---- */
 
 /* Kernel ids for fvx_profile_read. */
