@@ -351,8 +351,47 @@ fvx_status fvx_batch_steady_solve(fvx_batch b, const fvx_solver_cfg* cfg, double
 		require(b && cfg && u && stats, "null argument");
 		std::vector<fvx_solve_stats> v;
 		fvx::steady_core(*b->d, *cfg, u, b->ninst, b->d_gas.p, v,
-		                 static_cast<hipStream_t>(stream));
+		                 static_cast<hipStream_t>(stream),
+		                 b->norms.empty() ? nullptr : b->norms.data());
 		for(int i = 0; i < b->ninst; i++) stats[i] = v[i];
+	});
+}
+
+fvx_status fvx_batch_set_norms(fvx_batch b, const double* norms)
+{
+	return guarded([&] {
+		require(b != nullptr, "null handle");
+		if(!norms) { b->norms.clear(); return; }
+		for(int i = 0; i < b->ninst; i++)
+			require(norms[i] > 0 && std::isfinite(norms[i]), "normalisers must be positive");
+		b->norms.assign(norms, norms + b->ninst);
+	});
+}
+
+fvx_status fvx_batch_residual_norms(fvx_batch b, const double* u, double* out, void* stream)
+{
+	return guarded([&] {
+		require(b && u && out, "null argument");
+		hipStream_t s = static_cast<hipStream_t>(stream);
+		fvx_disc_s& d = *b->d;
+		const int n = d.m->nelem;
+		fvx::DBuf<double> r, part, nrm;
+		r.alloc((size_t)4*n*b->ninst);
+		part.alloc((size_t)fvx::partial_stride()*b->ninst);
+		nrm.alloc(b->ninst);
+		FVX_HIP(hipMemsetAsync(r.p, 0, r.n*sizeof(double), s));
+		fvx::BatchArg ba;
+		ba.list = b->d_all.p;
+		ba.gp = b->d_gas.p;
+		fvx::launch_residual(d, u, r.p, 0, nullptr, s, ba, b->ninst, b->d_grad.p, b->d_rgrad.p);
+		fvx::Inst L;
+		L.list = b->d_all.p;
+		L.nact = b->ninst;
+		fvx::launch_resnorm(n, r.p, d.m->d_area.p, part.p, nrm.p, 1, L, s);
+		std::vector<double> h(b->ninst);
+		FVX_HIP(hipMemcpyAsync(h.data(), nrm.p, h.size()*sizeof(double), hipMemcpyDeviceToHost, s));
+		FVX_HIP(hipStreamSynchronize(s));
+		for(int i = 0; i < b->ninst; i++) out[i] = std::sqrt(h[i]);
 	});
 }
 

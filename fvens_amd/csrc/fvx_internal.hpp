@@ -157,6 +157,7 @@ struct fvx_batch_s {
 	fvx::DBuf<int> d_all;                  // 0 .. ninst-1
 	fvx::DBuf<double> d_grad, d_rgrad;     // [ninst][nelem][8]
 	fvx::DBuf<double> d_scratch;           // functional partial sums
+	std::vector<double> norms;             // fixed convergence normalisers (empty: none)
 };
 
 /// Block matrix on the cell graph: vals[cell][slot][4][4], slot 0 the diagonal block and slot
@@ -263,8 +264,11 @@ int linear_solve(const fvx_matrix_s& A, const fvx_pc_s& p, const double* b, doub
                  int restart, double rtol, hipStream_t s);
 void steady_solve(fvx_disc_s& d, const fvx_solver_cfg& cfg, double* u, fvx_solve_stats& st,
                   hipStream_t s);
+/// Steady solve of ninst instances; norms (host, optional) are fixed per-instance normalisers
+/// of the convergence test (otherwise cfg.fixed_norm, or each instance's first residual norm).
 void steady_core(fvx_disc_s& d, const fvx_solver_cfg& c, double* u, int ninst,
-                 const GasParams* gp, std::vector<fvx_solve_stats>& stats, hipStream_t s);
+                 const GasParams* gp, std::vector<fvx_solve_stats>& stats, hipStream_t s,
+                 const double* norms = nullptr);
 
 // autotuner (tune.cpp)
 void autotune_launch(fvx_disc_s& d, const double* u, int reps, std::vector<fvx_tune_row>& rows,

@@ -160,7 +160,7 @@ struct Krylov {
 	long N = 0;
 	int m = 0, cap = 0;
 	DBuf<double> V, Z, partial, scal;
-	DBuf<int> dlist;
+	DBuf<int> dlist, dlist_in;      // instance lists of the caller's launches / of solve()
 	double* hpin = nullptr;
 	int* ipin = nullptr;
 	static constexpr int H0 = 0, NRM = 34, BETA = 35, XX = 36, RES = 37, Y0 = 40, NSCAL = 80;
@@ -177,6 +177,7 @@ struct Krylov {
 		partial.alloc((size_t)cap*partial_stride());
 		scal.alloc((size_t)cap*NSCAL);
 		dlist.alloc(cap);
+		dlist_in.alloc(cap);
 		FVX_HIP(hipHostMalloc((void**)&hpin, (size_t)cap*NSCAL*sizeof(double)));
 		FVX_HIP(hipHostMalloc((void**)&ipin, (size_t)cap*sizeof(int)));
 		std::memset(hpin, 0, (size_t)cap*NSCAL*sizeof(double));
@@ -196,14 +197,16 @@ struct Krylov {
 
 	/// Launch description for the instances `ids`: no list for a single-instance solve,
 	/// otherwise the list uploaded to the device (the upload is waited for, so the pinned
-	/// staging buffer can be reused at once).
-	Inst inst(const std::vector<int>& ids, bool single, hipStream_t s)
+	/// staging buffer can be reused at once). The caller's lists and those of solve() live in
+	/// separate device buffers, so a list the caller holds stays valid across solve().
+	Inst inst(const std::vector<int>& ids, bool single, hipStream_t s, bool internal = false)
 	{
 		if(single) return Inst{nullptr, 1};
+		int* dst = internal ? dlist_in.p : dlist.p;
 		std::memcpy(ipin, ids.data(), ids.size()*sizeof(int));
-		FVX_HIP(hipMemcpyAsync(dlist.p, ipin, ids.size()*sizeof(int), hipMemcpyHostToDevice, s));
+		FVX_HIP(hipMemcpyAsync(dst, ipin, ids.size()*sizeof(int), hipMemcpyHostToDevice, s));
 		FVX_HIP(hipStreamSynchronize(s));
-		return Inst{dlist.p, (int)ids.size()};
+		return Inst{dst, (int)ids.size()};
 	}
 
 	/// Device scalar slots of every instance to the host (one copy, then waits).
@@ -222,7 +225,7 @@ struct Krylov {
 	           const std::vector<int>& ids, bool single, std::vector<int>& its, hipStream_t s)
 	{
 		its.assign(cap, 0);
-		Inst L = inst(ids, single, s);
+		Inst L = inst(ids, single, s, true);
 		launch_mdot(N, b, 0, N, 1, b, N, partial.p, scal.p + BETA, NSCAL, L, s);
 		fetch(s);
 		struct St { std::vector<double> H, cs, sn, g; double beta = 0; int it = 0; };
@@ -243,7 +246,7 @@ struct Krylov {
 		}
 		if(act.empty()) { FVX_HIP(hipStreamSynchronize(s)); return; }
 		const std::vector<int> started = act;
-		if(act.size() != ids.size()) L = inst(act, single, s);
+		if(act.size() != ids.size()) L = inst(act, single, s, true);
 		launch_scale(N, b, N, scal.p + BETA, NSCAL, V.p, sV(), L, s);
 		for(int j = 0; j < mm && !act.empty(); j++) {
 			double* vj = V.p + (size_t)j*N;
@@ -282,7 +285,7 @@ struct Krylov {
 				t.it = j + 1;
 				if(!(std::fabs(t.g[j + 1]) <= rtol*t.beta || hn == 0.0)) next.push_back(i);
 			}
-			if(!next.empty() && next.size() != act.size()) L = inst(next, single, s);
+			if(!next.empty() && next.size() != act.size()) L = inst(next, single, s, true);
 			act.swap(next);
 		}
 		// solution weights, then x = sum_k y_k Z_k for the instances with equal counts together
@@ -308,7 +311,7 @@ struct Krylov {
 			size_t k1 = k0;
 			while(k1 < done.size() && its[done[k1]] == its[done[k0]]) k1++;
 			const std::vector<int> grp(done.begin() + k0, done.begin() + k1);
-			const Inst G = inst(grp, single, s);
+			const Inst G = inst(grp, single, s, true);
 			launch_lincomb(N, Z.p, N, sZ(), its[grp[0]], scal.p + Y0, NSCAL, x, N, G, s);
 			k0 = k1;
 		}
@@ -402,7 +405,8 @@ static void validate(const fvx_disc_s& d, const fvx_solver_cfg& c)
 }
 
 void steady_core(fvx_disc_s& d, const fvx_solver_cfg& c, double* u, int ninst,
-                 const GasParams* gp, std::vector<fvx_solve_stats>& stats, hipStream_t s)
+                 const GasParams* gp, std::vector<fvx_solve_stats>& stats, hipStream_t s,
+                 const double* norms)
 {
 	validate(d, c);
 	const fvx_mesh_s& m = *d.m;
@@ -444,7 +448,8 @@ void steady_core(fvx_disc_s& d, const fvx_solver_cfg& c, double* u, int ninst,
 	struct Ist { double cfl = 0, resi = 1, resiold = 1, initres = 1; int step = 0; };
 	std::vector<Ist> is(ninst);
 	stats.assign(ninst, fvx_solve_stats{});
-	for(auto& t : is) t.initres = c.fixed_norm > 0 ? c.fixed_norm : 1.0;
+	for(int i = 0; i < ninst; i++)
+		is[i].initres = norms ? norms[i] : c.fixed_norm > 0 ? c.fixed_norm : 1.0;
 	auto active = [&](int i) {
 		return stats[i].status == 0 && is[i].resi/is[i].initres > c.tol && is[i].step < c.maxiter;
 	};
@@ -543,7 +548,7 @@ void steady_core(fvx_disc_s& d, const fvx_solver_cfg& c, double* u, int ninst,
 		for(int i : act) {
 			if(!std::isfinite(is[i].resi)) stats[i].status = FVX_ERR_NONFINITE;
 			else {
-				if(is[i].step == 0 && c.fixed_norm <= 0) is[i].initres = is[i].resi;
+				if(is[i].step == 0 && !norms && c.fixed_norm <= 0) is[i].initres = is[i].resi;
 				is[i].step++;
 			}
 			if(!active(i)) t_end[i] = now();

@@ -459,6 +459,23 @@ class Batch:
                                            _stream()))
         return [{f: getattr(s, f) for f, _ in SolveStats._fields_} for s in st]
 
+    def set_norms(self, norms=None):
+        """Fixed per-instance normalisers of the convergence test (None: first-step norms)."""
+        if norms is None:
+            check(lib().fvx_batch_set_norms(self.h, C.c_void_p(0)))
+            return
+        norms = np.ascontiguousarray(norms, np.float64)
+        if norms.shape != (self.ninst,):
+            raise ValueError("one normaliser per instance")
+        check(lib().fvx_batch_set_norms(self.h, _ptr(norms, C.c_double)))
+
+    def residual_norms(self, u):
+        """sqrt(sum r_E^2 area) of every instance at u."""
+        out = np.zeros(self.ninst)
+        check(lib().fvx_batch_residual_norms(self.h, _dptr(u, 4 * self.ninst * self.disc.mesh.nelem),
+                                             _ptr(out, C.c_double), _stream()))
+        return out
+
     def functionals(self, u, wall_tag=2):
         out = np.zeros((self.ninst, 4))
         check(lib().fvx_batch_functionals(self.h, _dptr(u, 4 * self.ninst * self.disc.mesh.nelem),

@@ -82,3 +82,76 @@ def train(net: "fvx.MPNN", targets, mach, aoa_deg, steps: int, graphs_per_step: 
             history.append((it, loss))
             log(f"step {it:6d} loss {loss:.4e} ({time.perf_counter() - t0:.1f} s)")
     return history
+
+
+def freestream_states(batch: "fvx.Batch"):
+    """[ninst, ncell, 4] device tensor of every instance's freestream state."""
+    import torch
+    fs = torch.tensor(batch.freestream(), device="cuda")
+    return fs[:, None, :].expand(batch.ninst, batch.disc.mesh.nelem, 4).contiguous()
+
+
+def solve_batch(disc2: "fvx.FlowFV", mach, aoa_deg, u=None, starter: "fvx.FlowFV" = None,
+                starter_cfg=None, main_cfg=None, norms=None):
+    """Steady solves of the points (mach[i], aoa_deg[i]) in one batch: an optional first-order
+    starter solve (the reference's case driver, src/utilities/casesolvers.cpp:225-386) then the
+    main solve on disc2. u: initial states (default freestream), updated in place and returned
+    with per-instance statistics and the wall time of each phase."""
+    import torch
+    b2 = fvx.Batch(disc2, mach, aoa_deg)
+    if u is None:
+        u = freestream_states(b2)
+    if norms is not None:
+        b2.set_norms(norms)
+    out = {}
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    if starter is not None:
+        b1 = fvx.Batch(starter, mach, aoa_deg)
+        if norms is not None:
+            b1.set_norms(norms)
+        out["starter"] = b1.steady_solve(u, **starter_cfg)
+        torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    out["main"] = b2.steady_solve(u, **main_cfg)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    out["t_starter"], out["t_main"] = t1 - t0, t2 - t1
+    out["functionals"] = b2.functionals(u)
+    return u, out
+
+
+def generate_targets(net: "fvx.MPNN", disc2, starter, mach, aoa_deg, starter_cfg, main_cfg,
+                     chunk: int = 64, log=print):
+    """Training set: converged fields of the batched solver for every point, turned into network
+    targets [npoints, ncell, 4] (fp32, device). Returns the targets and per-point statistics."""
+    import torch
+    n = len(mach)
+    T = torch.zeros((n, net.N, 4), dtype=torch.float32, device="cuda")
+    stats = []
+    for c0 in range(0, n, chunk):
+        sl = slice(c0, min(n, c0 + chunk))
+        u, st = solve_batch(disc2, mach[sl], aoa_deg[sl], starter=starter,
+                            starter_cfg=starter_cfg, main_cfg=main_cfg)
+        for s0 in range(sl.start, sl.stop, net.max_graphs):
+            ss = slice(s0, min(sl.stop, s0 + net.max_graphs))
+            net.set_points(mach[ss], aoa_deg[ss])
+            net.targets(u[ss.start - c0:ss.stop - c0], T[ss], ngraph=ss.stop - ss.start)
+        for i, s in enumerate(st["main"]):
+            stats.append(dict(mach=float(mach[c0 + i]), aoa=float(aoa_deg[c0 + i]),
+                              steps=s["steps"], status=s["status"], final_rel=s["final_rel"],
+                              starter_steps=st["starter"][i]["steps"] if starter else 0))
+        log(f"targets {sl.stop}/{n}: starter {st['t_starter']:.1f} s, main {st['t_main']:.1f} s, "
+            f"converged {sum(s['status'] == 0 for s in st['main'])}/{sl.stop - sl.start}")
+    return T, stats
+
+
+def warm_states(net: "fvx.MPNN", mach, aoa_deg, floor: float = 0.05):
+    """Predicted conserved states [npoints, ncell, 4] (device, float64)."""
+    import torch
+    u = torch.zeros((len(mach), net.N, 4), dtype=torch.float64, device="cuda")
+    for c0 in range(0, len(mach), net.max_graphs):
+        sl = slice(c0, min(len(mach), c0 + net.max_graphs))
+        net.set_points(mach[sl], aoa_deg[sl])
+        net.warm_state(u[sl], ngraph=sl.stop - sl.start, floor=floor)
+    return u

@@ -242,6 +242,14 @@ fvx_status fvx_batch_compute_residual(fvx_batch b, const double* u, double* r, i
  * fails. */
 fvx_status fvx_batch_steady_solve(fvx_batch b, const fvx_solver_cfg* cfg, double* u,
                                   fvx_solve_stats* stats, void* stream);
+/* Fixed normalisers of the convergence test, one per instance (host norms[ninst]): instance i
+ * has converged when its residual norm falls to cfg.tol x norms[i]. NULL restores the
+ * reference's normaliser, the norm of the first step (src/ode/aodesolver.cpp:536-537). No
+ * reference counterpart (a fixed normaliser makes warm and cold starts comparable). */
+fvx_status fvx_batch_set_norms(fvx_batch b, const double* norms);
+/* The convergence norm sqrt(sum_cells r_energy^2 area) of every instance at u (the reference's
+ * residual norm, src/ode/aodesolver.cpp:516-530), host out[ninst]. */
+fvx_status fvx_batch_residual_norms(fvx_batch b, const double* u, double* out, void* stream);
 /* fvx_functionals for every instance, host out[ninst][4]. */
 fvx_status fvx_batch_functionals(fvx_batch b, const double* u, int wall_tag, double* out,
                                  void* stream);

@@ -57,7 +57,8 @@ def test_batched_solve_equals_single_solves(torch_cuda, kw):
     gm = fvx.Mesh(raw)
     base = _single(gm, 0.5, 0.0)
     b = fvx.Batch(base, MACH, AOA)
-    solve = dict(cflinit=100.0, cflfin=1000.0, tol=1e-6, maxiter=60, min_relax=0.2)
+    # a tolerance the instances reach at different steps (retirement is exercised)
+    solve = dict(cflinit=100.0, cflfin=1000.0, tol=1e-4, maxiter=80, min_relax=0.2)
     solve.update(kw)
     if not solve.get("implicit", True):
         solve.update(maxiter=25, tol=1e-3)

@@ -63,7 +63,7 @@ def test_graph_features_match(setup):
     f = net.features()
     assert np.array_equal(f["nbr"], g["nbr"])
     assert np.array_equal(f["rev"], g["rev"])
-    assert np.array_equal(f["invdeg"], g["invdeg"])
+    assert np.array_equal(f["invdeg"], g["inv_deg"])
     np.testing.assert_allclose(f["stat"], g["static"], rtol=1e-6, atol=1e-6)
     np.testing.assert_allclose(f["edge"], g["edge"], rtol=1e-6, atol=1e-6)
     # the wall-adjacent flag marks exactly the cells with a wall face
