@@ -38,6 +38,13 @@ def parse():
     p.add_argument("--cpu-seconds", type=float, default=12.0,
                    help="bounded CPU-baseline sample (seconds of CPU work)")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--no-batch", action="store_true",
+                   help="skip the batched operating-point sweep")
+    p.add_argument("--batch-instances", type=int, default=64)
+    p.add_argument("--batch-maxiter", type=int, default=200)
+    p.add_argument("--warm-params", default=os.path.join(ROOT, "fvens_amd", "data",
+                                                          "warmstart_k2.npz"))
+    p.add_argument("--warm-points", type=int, default=32)
     p.add_argument("--no-solve", action="store_true",
                    help="skip the time-to-tolerance pseudo-time solves")
     p.add_argument("--calibrate-copy", action="store_true",
@@ -163,6 +170,87 @@ def cpu_time_to_tol(name, tol):
                 CL=f["CL"], CDp=f["CDp"], threads=1, pc="natural-order block ILU(0)")
 
 
+def batch_sweep(ninst, maxiter, sweeps=16, tol=1e-8):
+    """Batched operating-point sweep on the 98,304-cell airfoil: ninst seeded points
+    (std::mt19937_64 seed 4242, Mach U[0.3, 0.75], AoA U[-4, 6] deg), Roe + WLS +
+    Venkatakrishnan K=5, first-order starter then the second-order matrix-free solve, FP32
+    Jacobian and preconditioner, FP64 residual, retirement at tol x each point's freestream
+    residual norm."""
+    import numpy as np
+    import torch
+    from fvens_amd import fvx, meshgen, train_warmstart as tw, warmstart as ws
+    from fvens_amd.points import operating_points
+    raw = meshgen.mesh_k2()
+    raw = raw.permuted(fvx.rcm_permutation(raw))
+    gm = fvx.Mesh(raw)
+    d1, d2 = tw.discretisations(gm)
+    pc32, _, starter, main = tw.solver_settings(sweeps, maxiter, tol)
+    mach, aoa = operating_points(4242, ninst)
+    b = fvx.Batch(d2, mach, aoa)
+    norms = b.residual_norms(ws.freestream_states(b))
+    u, o = ws.solve_batch(d2, mach, aoa, starter=d1, starter_cfg=dict(starter, **pc32),
+                          main_cfg=dict(main, **pc32), norms=norms)
+    wall = o["t_starter"] + o["t_main"]
+    st1 = [s["steps"] for s in o["starter"]]
+    st2 = [s["steps"] for s in o["main"]]
+    status = [s["status"] for s in o["main"]]
+    cell_iters = raw.nelem * (sum(st1) + sum(st2))
+    del u
+    torch.cuda.empty_cache()
+    return {
+        "case": f"{ninst} operating points on the 98,304-cell airfoil, Roe, WLS, "
+                f"Venkatakrishnan K=5, FP32 Jacobian + multicolour SGS x{sweeps}, FP64 residual",
+        "value": round(cell_iters / wall / 1e6, 2), "unit": "Mcell-iterations/s (aggregate)",
+        "seconds": round(wall, 3), "starter_seconds": round(o["t_starter"], 3),
+        "tol": tol, "maxiter": maxiter,
+        "converged": int(sum(1 for x in status if x == 0)),
+        "not_converged": int(sum(1 for x in status if x == 4)),
+        "nonfinite": int(sum(1 for x in status if x == 3)),
+        "starter_steps": st1, "main_steps": st2, "status": status,
+        "final_rel": [float("%.3e" % s["final_rel"]) for s in o["main"]],
+        "lin_iters": [s["lin_iters"] for s in o["main"]],
+    }
+
+
+def warm_start_ab(params_path, ntest, nsingle, maxiter, tol=1e-8, sweeps=16):
+    """A/B of the learned warm start on held-out points (seed 200): plain FP64 solve from the
+    freestream with the first-order starter against the network's prediction with FP32
+    Jacobian + preconditioner, both to tol x the freestream residual norm, with fallback."""
+    import numpy as np
+    from fvens_amd import fvx, meshgen, train_warmstart as tw
+    from fvens_amd.points import operating_points
+    raw = meshgen.mesh_k2()
+    raw = raw.permuted(fvx.rcm_permutation(raw))
+    gm = fvx.Mesh(raw)
+    d1, d2 = tw.discretisations(gm)
+    pc32, pc64, starter, main = tw.solver_settings(sweeps, maxiter, tol)
+    net = fvx.MPNN(gm, 2, 16)
+    net.set_params(np.load(params_path)["params"])
+    mach, aoa = operating_points(200, ntest)
+    quiet = lambda s: None  # noqa: E731
+    ab = tw.ab_test(net, d1, d2, mach, aoa, pc32, pc64, starter, main, quiet)
+    single = tw.single_ab(net, d1, d2, mach[:nsingle], aoa[:nsingle], pc32, pc64, starter,
+                          main, quiet)
+    pts = ab["points"]
+    ok = [p for p in pts if p["cold_status"] == 0 and p["warm_status"] == 0]
+    return {
+        "case": f"{ntest} held-out points (seed 200) on the 98,304-cell airfoil, Roe, WLS, "
+                f"Venkatakrishnan K=5; tol {tol:g} x freestream residual norm",
+        "batched": {"plain_fp64_seconds": round(ab["t_cold"], 3),
+                    "warm_fp32_seconds": round(ab["t_warm"] + ab["t_fallback"], 3),
+                    "speedup": round(ab["speedup_batched"], 3), "fallbacks": ab["fallbacks"],
+                    "plain_steps": [p["cold_steps"] for p in pts],
+                    "warm_steps": [p["warm_steps"] for p in pts],
+                    "both_converged": len(ok),
+                    "max_dCL": max([p["dCL"] for p in ok], default=None),
+                    "max_dCDp": max([p["dCDp"] for p in ok], default=None)},
+        "single": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+                   for r in single],
+        "single_speedup": round(sum(r["t_cold"] for r in single)
+                                / max(1e-9, sum(r["t_warm"] for r in single)), 3),
+    }
+
+
 def cpu_baseline(raw, bcs, u, budget_s):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
@@ -279,7 +367,14 @@ def main():
             tt["bump_1e-10"]["cpu_oracle_fp64"] = cpu_time_to_tol("bump", 1e-10)
             tt["bump_1e-10"]["gpu_vs_cpu"] = round(
                 tt["bump_1e-10"]["cpu_oracle_fp64"]["seconds"] / bump64["seconds"], 2)
+        if os.path.exists(a.warm_params):
+            tt["warm_start_airfoil"] = warm_start_ab(a.warm_params, a.warm_points, 4,
+                                                     a.batch_maxiter)
+        else:
+            tt["warm_start_airfoil"] = {"skipped": "no trained parameters at " + a.warm_params}
         out["time_to_tol"] = tt
+    if not a.no_batch:
+        out["batch"] = batch_sweep(a.batch_instances, a.batch_maxiter)
     print(json.dumps(out), flush=True)
 
 

@@ -131,8 +131,11 @@ def generate_targets(net: "fvx.MPNN", disc2, starter, mach, aoa_deg, starter_cfg
     stats = []
     for c0 in range(0, n, chunk):
         sl = slice(c0, min(n, c0 + chunk))
-        u, st = solve_batch(disc2, mach[sl], aoa_deg[sl], starter=starter,
-                            starter_cfg=starter_cfg, main_cfg=main_cfg)
+        b = fvx.Batch(disc2, mach[sl], aoa_deg[sl])
+        u0 = freestream_states(b)
+        norms = b.residual_norms(u0)
+        u, st = solve_batch(disc2, mach[sl], aoa_deg[sl], u=u0, starter=starter,
+                            starter_cfg=starter_cfg, main_cfg=main_cfg, norms=norms)
         for s0 in range(sl.start, sl.stop, net.max_graphs):
             ss = slice(s0, min(sl.stop, s0 + net.max_graphs))
             net.set_points(mach[ss], aoa_deg[ss])
