@@ -45,6 +45,8 @@ def parse():
     p.add_argument("--warm-params", default=os.path.join(ROOT, "fvens_amd", "data",
                                                           "warmstart_k2.npz"))
     p.add_argument("--warm-points", type=int, default=32)
+    p.add_argument("--only", choices=["residual", "solve", "batch", "mpnn"], default=None,
+                   help="run one leg only (for profiling); the JSON line then lacks the others")
     p.add_argument("--no-solve", action="store_true",
                    help="skip the time-to-tolerance pseudo-time solves")
     p.add_argument("--calibrate-copy", action="store_true",
@@ -244,11 +246,51 @@ def warm_start_ab(params_path, ntest, nsingle, maxiter, tol=1e-8, sweeps=16):
                     "both_converged": len(ok),
                     "max_dCL": max([p["dCL"] for p in ok], default=None),
                     "max_dCDp": max([p["dCDp"] for p in ok], default=None)},
-        "single": [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+        "single": [{k: (float("%.4g" % v) if isinstance(v, float) else v) for k, v in r.items()}
                    for r in single],
         "single_speedup": round(sum(r["t_cold"] for r in single)
                                 / max(1e-9, sum(r["t_warm"] for r in single)), 3),
     }
+
+
+def mpnn_timing(graphs, steps):
+    """Device time of one training step (forward, backward, Adam) and of one inference pass of
+    the warm-start network over `graphs` fields of the 98,304-cell airfoil graph (synthetic
+    targets), measured with HIP events on the launch stream."""
+    import numpy as np
+    import torch
+    from fvens_amd import fvx, meshgen, warmstart as ws
+    raw = meshgen.mesh_k2()
+    raw = raw.permuted(fvx.rcm_permutation(raw))
+    gm = fvx.Mesh(raw)
+    net = fvx.MPNN(gm, 2, graphs)
+    net.set_params(ws.init_params(0))
+    rng = np.random.default_rng(0)
+    T = torch.tensor((rng.standard_normal((graphs, raw.nelem, 4)) * 0.1).astype(np.float32),
+                     device="cuda")
+    net.set_points(np.linspace(0.3, 0.75, graphs), np.linspace(-4, 6, graphs))
+    ids = np.arange(graphs)
+    y = torch.zeros((graphs, raw.nelem, 4), dtype=torch.float32, device="cuda")
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+    for _ in range(3):
+        net.train_step(T, ids, 1e-4, want_loss=False)
+        net.predict(y)
+    ev[0].record()
+    for _ in range(steps):
+        net.train_step(T, ids, 1e-4, want_loss=False)
+    ev[1].record()
+    ev[2].record()
+    for _ in range(steps):
+        net.predict(y)
+    ev[3].record()
+    torch.cuda.synchronize()
+    tr = ev[0].elapsed_time(ev[1]) / steps
+    inf = ev[2].elapsed_time(ev[3]) / steps
+    rows = graphs * raw.nelem
+    return {"case": f"4-layer width-64 message passing over {graphs} fields of the "
+                    f"98,304-cell airfoil graph ({rows} rows), bf16 MFMA, fp32 accumulation",
+            "train_step_ms": round(tr, 4), "inference_ms": round(inf, 4),
+            "train_rows_per_s": round(rows / tr * 1e3), "infer_rows_per_s": round(rows / inf * 1e3)}
 
 
 def cpu_baseline(raw, bcs, u, budget_s):
@@ -284,6 +326,16 @@ def main():
     if a.calibrate_copy:
         calibrate_copy(a.steps)
         return
+    if a.only == "batch":
+        print(json.dumps({"batch": batch_sweep(a.batch_instances, a.batch_maxiter)}), flush=True)
+        return
+    if a.only == "mpnn":
+        print(json.dumps({"mpnn": mpnn_timing(8, min(a.steps, 50))}), flush=True)
+        return
+    if a.only == "solve":
+        a.no_batch = True
+    elif a.only == "residual":
+        a.no_batch = a.no_solve = True
     raw, bcs, pcfg, ncfg, u_host = build_case()
     mesh = fvx.Mesh(raw)
     disc = fvx.FlowFV(mesh, pcfg, ncfg)
@@ -375,6 +427,7 @@ def main():
         out["time_to_tol"] = tt
     if not a.no_batch:
         out["batch"] = batch_sweep(a.batch_instances, a.batch_maxiter)
+        out["mpnn"] = mpnn_timing(8, 20)
     print(json.dumps(out), flush=True)
 
 

@@ -625,17 +625,27 @@ __global__ void __launch_bounds__(RB) k_wgrad(int R, int chunks_per_block,
 	if(BIAS && tid < NO) out[K*NO + tid] = bsum;
 }
 
-/// Sums `nblk` partials of `width` values in block order into the flat gradient. Modes:
+/// Sums `nblk` partials of `width` values into the flat gradient. A workgroup owns 16
+/// consecutive values; its 16 lane groups each sum every 16th partial (in increasing order), and
+/// the 16 group sums are added in group order, so the result is deterministic. Modes:
 /// 0 contiguous at off1; 1 columns split (a [rows][width/rows] matrix whose left half goes to
 /// off1 and right half to off2); 2 rows split (rows < half -> off1, the rest -> off2).
+constexpr int RED_COLS = 16, RED_GROUPS = RB/RED_COLS;
 __global__ void __launch_bounds__(RB) k_reduce(int nblk, int width, int ncols, int mode,
                                                int off1, int off2, const float* __restrict__ part,
                                                float* __restrict__ grad, float* __restrict__ extra)
 {
-	const int i = blockIdx.x*RB + threadIdx.x;
-	if(i >= width) return;
+	__shared__ double sh[RB];
+	const int c = threadIdx.x % RED_COLS, grp = threadIdx.x / RED_COLS;
+	const int i = blockIdx.x*RED_COLS + c;
+	double acc = 0.0;
+	if(i < width)
+		for(int b = grp; b < nblk; b += RED_GROUPS) acc += part[(size_t)b*width + i];
+	sh[threadIdx.x] = acc;
+	__syncthreads();
+	if(grp != 0 || i >= width) return;
 	double s = 0.0;
-	for(int b = 0; b < nblk; b++) s += part[(size_t)b*width + i];
+	for(int g = 0; g < RED_GROUPS; g++) s += sh[g*RED_COLS + c];
 	const int k = i / ncols, n = i % ncols;
 	if(mode == 0) {
 		if(extra && i == width - 1) { *extra = (float)s; return; }
@@ -720,6 +730,7 @@ double point_segment(double px, double py, double ax, double ay, double bx, doub
 }
 
 int blocks_for(size_t n) { return (int)((n + RB - 1)/RB); }
+int reduce_grid(int width) { return (width + RED_COLS - 1)/RED_COLS; }
 
 }
 
@@ -897,7 +908,7 @@ void Mpnn::gradient(int G, const float* targets, const int* sample_ids, hipStrea
 	float* gn = g1.p;
 	k_decode<<<RED_BLOCKS, RB, 0, s>>>(g, h[NL].p, wq.p, nullptr, targets, ids.p,
 	                                   1.0/((double)R*FO), gc, part.p);
-	k_reduce<<<blocks_for(261), RB, 0, s>>>(RED_BLOCKS, 261, 261, 0, OFF_WDEC, 0, part.p, grad.p,
+	k_reduce<<<reduce_grid(261), RB, 0, s>>>(RED_BLOCKS, 261, 261, 0, OFF_WDEC, 0, part.p, grad.p,
 	                                        loss.p);
 	const int nchunk = (R + CH - 1)/CH;
 	const int nwb = std::min(RED_BLOCKS, nchunk);
@@ -910,19 +921,19 @@ void Mpnn::gradient(int G, const float* targets, const int* sample_ids, hipStrea
 		k_update_bwd<<<gemm_grid(R), RB, 0, s>>>(R, gc, umask[l].p, fl + M_BUPD*FRAG_MAT, dpre.p,
 		                                         gpart.p, da.p);
 		k_message_bwd<<<RED_BLOCKS, RB, 0, s>>>(g, pq[l].p, da.p, wl, dpq.p, part.p);
-		k_reduce<<<blocks_for(320), RB, 0, s>>>(RED_BLOCKS, 320, 320, 0, base + L_WE, 0, part.p,
+		k_reduce<<<reduce_grid(320), RB, 0, s>>>(RED_BLOCKS, 320, 320, 0, base + L_WE, 0, part.p,
 		                                        grad.p, nullptr);
 		k_pq_bwd<<<gemm_grid(R), RB, 0, s>>>(R, dpq.p, fl + M_BPQ*FRAG_MAT, gpart.p, gn);
 		k_wgrad<W, 2*W, false><<<nwb_used, RB, 0, s>>>(R, cpb, h[l].p, nullptr, dpq.p, part.p);
-		k_reduce<<<blocks_for(W*2*W), RB, 0, s>>>(nwb_used, W*2*W, 2*W, 1, base + L_WA, base + L_WB,
+		k_reduce<<<reduce_grid(W*2*W), RB, 0, s>>>(nwb_used, W*2*W, 2*W, 1, base + L_WA, base + L_WB,
 		                                          part.p, grad.p, nullptr);
 		k_wgrad<2*W, W, true><<<nwb_used, RB, 0, s>>>(R, cpb, h[l].p, ab[l].p, dpre.p, part.p);
-		k_reduce<<<blocks_for(2*W*W + W), RB, 0, s>>>(nwb_used, 2*W*W + W, W, 2, base + L_WU,
+		k_reduce<<<reduce_grid(2*W*W + W), RB, 0, s>>>(nwb_used, 2*W*W + W, W, 2, base + L_WU,
 		                                              base + L_WV, part.p, grad.p, nullptr);
 		std::swap(gc, gn);
 	}
 	k_encode_bwd<<<RED_BLOCKS, RB, 0, s>>>(g, gc, part.p);
-	k_reduce<<<blocks_for(576), RB, 0, s>>>(RED_BLOCKS, 576, 576, 0, OFF_WENC, 0, part.p, grad.p,
+	k_reduce<<<reduce_grid(576), RB, 0, s>>>(RED_BLOCKS, 576, 576, 0, OFF_WENC, 0, part.p, grad.p,
 	                                        nullptr);
 	FVX_HIP(hipGetLastError());
 }
