@@ -109,41 +109,81 @@ def solve_with_fallback(net, d1, d2, mach, aoa, pc_warm, pc_plain, starter, main
     return u, out
 
 
-def ab_test(net, d1, d2, mach, aoa, pc32, pc64, starter, main, log, warm_cfl=1000.0):
-    """Batched A/B over the points plus per-point statistics."""
+def generate_targets_warm(net, d1, d2, mach, aoa, pc32, starter, main, chunk, log):
+    """Training targets of the points from warm-started solves with the current network (the
+    plain solve where a warm start fails): the network speeds up the generation of its own
+    next training set."""
     import torch
-    b = fvx.Batch(d2, mach, aoa)
-    norms = b.residual_norms(ws.freestream_states(b))
-    _, cold = ws.solve_batch(d2, mach, aoa, starter=d1, starter_cfg=dict(starter, **pc64),
-                             main_cfg=dict(main, **pc64), norms=norms)
-    start_ratio = b.residual_norms(ws.warm_states(net, mach, aoa)) / norms
-    _, warm = warm_solve(net, d1, d2, mach, aoa, pc32, main, norms, warm_cfl)
-    t_inf = warm["t_inference"]
-    failed = [i for i, s in enumerate(warm["main"]) if s["status"] != 0]
-    fb = None
-    if failed:
-        # fallback: the plain solve of the points whose warm start did not converge
-        _, fb = ws.solve_batch(d2, mach[failed], aoa[failed], starter=d1,
-                               starter_cfg=dict(starter, **pc64), main_cfg=dict(main, **pc64),
-                               norms=norms[failed])
+    n = len(mach)
+    T = torch.zeros((n, net.N, 4), dtype=torch.float32, device="cuda")
+    stats = []
+    for c0 in range(0, n, chunk):
+        sl = slice(c0, min(n, c0 + chunk))
+        m, a = mach[sl], aoa[sl]
+        b = fvx.Batch(d2, m, a)
+        norms = b.residual_norms(ws.freestream_states(b))
+        t0 = time.perf_counter()
+        u, res = solve_with_fallback(net, d1, d2, m, a, pc32, pc32, starter, main, norms)
+        torch.cuda.synchronize()
+        for s0 in range(0, len(m), net.max_graphs):
+            ss = slice(s0, min(len(m), s0 + net.max_graphs))
+            net.set_points(m[ss], a[ss])
+            net.targets(u[ss], T[sl.start + ss.start:sl.start + ss.stop], ngraph=ss.stop - ss.start)
+        for i, r in enumerate(res):
+            stats.append(dict(mach=float(m[i]), aoa=float(a[i]), steps=r["steps"],
+                              status=r["status"], final_rel=r["final_rel"], path=r["path"]))
+        log(f"targets {sl.stop}/{n} (warm-started): {time.perf_counter() - t0:.1f} s, "
+            f"fallbacks {sum(r['path'] == 'fallback' for r in res)}, "
+            f"converged {sum(r['status'] == 0 for r in res)}/{len(m)}")
+    return T, stats
+
+
+def ab_test(net, d1, d2, mach, aoa, pc32, pc64, starter, main, log, warm_cfl=1000.0,
+            chunk=64):
+    """Batched A/B over the points (in batches of `chunk`) plus per-point statistics."""
     pts = []
-    for i in range(len(mach)):
-        c, w = cold["main"][i], warm["main"][i]
-        fc, fw = cold["functionals"][i], warm["functionals"][i]
-        pts.append(dict(mach=float(mach[i]), aoa=float(aoa[i]),
-                        cold_steps=cold["starter"][i]["steps"] + c["steps"],
-                        cold_status=c["status"],
-                        warm_steps=warm["pre"][i]["steps"] + w["steps"], warm_status=w["status"],
-                        warm_start_residual_ratio=float(start_ratio[i]),
-                        dCL=abs(fw["CL"] - fc["CL"]), dCDp=abs(fw["CDp"] - fc["CDp"]),
-                        CL=fc["CL"], CDp=fc["CDp"]))
-    out = dict(t_cold=cold["t_starter"] + cold["t_main"],
-               t_warm=t_inf + warm["t_pre"] + warm["t_main"],
-               t_inference=t_inf, t_pre=warm["t_pre"], fallbacks=len(failed),
-               t_fallback=(fb["t_starter"] + fb["t_main"]) if fb else 0.0, points=pts)
+    tot = dict(t_cold=0.0, t_warm=0.0, t_inference=0.0, t_pre=0.0, t_fallback=0.0, fallbacks=0)
+    for c0 in range(0, len(mach), chunk):
+        m, a = mach[c0:c0 + chunk], aoa[c0:c0 + chunk]
+        b = fvx.Batch(d2, m, a)
+        norms = b.residual_norms(ws.freestream_states(b))
+        _, cold = ws.solve_batch(d2, m, a, starter=d1, starter_cfg=dict(starter, **pc64),
+                                 main_cfg=dict(main, **pc64), norms=norms)
+        log(f"A/B points {c0}..{c0 + len(m) - 1}: plain {cold['t_starter'] + cold['t_main']:.2f} s")
+        start_ratio = b.residual_norms(ws.warm_states(net, m, a)) / norms
+        _, warm = warm_solve(net, d1, d2, m, a, pc32, main, norms, warm_cfl)
+        failed = [i for i, s in enumerate(warm["main"]) if s["status"] != 0]
+        fb = None
+        if failed:
+            # fallback: the plain solve of the points whose warm start did not converge
+            _, fb = ws.solve_batch(d2, m[failed], a[failed], starter=d1,
+                                   starter_cfg=dict(starter, **pc64),
+                                   main_cfg=dict(main, **pc64), norms=norms[failed])
+        t_w = warm["t_inference"] + warm["t_pre"] + warm["t_main"]
+        t_f = (fb["t_starter"] + fb["t_main"]) if fb else 0.0
+        log(f"A/B points {c0}..{c0 + len(m) - 1}: warm {t_w:.2f} s + fallback {t_f:.2f} s "
+            f"({len(failed)} points)")
+        tot["t_cold"] += cold["t_starter"] + cold["t_main"]
+        tot["t_warm"] += t_w
+        tot["t_inference"] += warm["t_inference"]
+        tot["t_pre"] += warm["t_pre"]
+        tot["t_fallback"] += t_f
+        tot["fallbacks"] += len(failed)
+        for i in range(len(m)):
+            c, w = cold["main"][i], warm["main"][i]
+            fc, fw = cold["functionals"][i], warm["functionals"][i]
+            pts.append(dict(mach=float(m[i]), aoa=float(a[i]),
+                            cold_steps=cold["starter"][i]["steps"] + c["steps"],
+                            cold_status=c["status"],
+                            warm_steps=warm["pre"][i]["steps"] + w["steps"],
+                            warm_status=w["status"],
+                            warm_start_residual_ratio=float(start_ratio[i]),
+                            dCL=abs(fw["CL"] - fc["CL"]), dCDp=abs(fw["CDp"] - fc["CDp"]),
+                            CL=fc["CL"], CDp=fc["CDp"]))
+    out = dict(tot, points=pts)
     out["speedup_batched"] = out["t_cold"] / (out["t_warm"] + out["t_fallback"])
     log(f"A/B batched: plain {out['t_cold']:.2f} s, warm {out['t_warm']:.2f} s "
-        f"(+{out['t_fallback']:.2f} s fallback for {len(failed)}), "
+        f"(+{out['t_fallback']:.2f} s fallback for {out['fallbacks']}), "
         f"speedup {out['speedup_batched']:.2f}")
     return out
 
@@ -188,6 +228,12 @@ def main():
     ap.add_argument("--chunk", type=int, default=64)
     ap.add_argument("--params", default=None, help="start from these parameters (npz)")
     ap.add_argument("--warm-cfl", type=float, default=1000.0)
+    ap.add_argument("--skip", type=int, default=0,
+                    help="use training draws skip .. skip+ntrain-1 of the seed-2000 sequence")
+    ap.add_argument("--warm-labels", action="store_true",
+                    help="generate the training fields with warm starts from --params")
+    ap.add_argument("--from-scratch", action="store_true",
+                    help="train from the initial parameters even when --params is given")
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--out", default=".")
     a = ap.parse_args()
@@ -205,18 +251,28 @@ def main():
     net = fvx.MPNN(gm, 2, a.graphs)
     res = dict(args=vars(a), cells=raw.nelem, nparams=net.nparams())
     if a.ntrain > 0:
-        mtr, atr = operating_points(2000, a.ntrain)
+        mtr, atr = operating_points(2000, a.skip + a.ntrain)
+        mtr, atr = mtr[a.skip:], atr[a.skip:]
         t0 = time.perf_counter()
         label = dict(main_cfg, tol=a.label_tol)
-        T, st = ws.generate_targets(net, d2, d1, mtr, atr, dict(starter, **pc32),
-                                    dict(label, **pc32), chunk=a.chunk, log=log)
+        if a.warm_labels:
+            net.set_params(np.load(a.params)["params"])
+            T, st = generate_targets_warm(net, d1, d2, mtr, atr, pc32, starter, label, a.chunk,
+                                          log)
+        else:
+            T, st = ws.generate_targets(net, d2, d1, mtr, atr, dict(starter, **pc32),
+                                        dict(label, **pc32), chunk=a.chunk, log=log)
         res["t_data"] = time.perf_counter() - t0
         res["train_points"] = st
-        net.set_params(ws.init_params(0) if a.params is None
+        valid = ws.usable(st)
+        res["train_fields_used"] = len(valid)
+        log(f"training on {len(valid)} of {len(st)} fields (non-finite solves dropped)")
+        net.set_params(ws.init_params(0) if a.params is None or a.from_scratch
                        else np.load(a.params)["params"])
         t0 = time.perf_counter()
         res["history"] = ws.train(net, T, mtr, atr, steps=a.steps, graphs_per_step=a.graphs,
-                                  lr0=a.lr, log_every=max(1, a.steps // 20), log=log)
+                                  lr0=a.lr, log_every=max(1, a.steps // 20), log=log,
+                                  valid=valid)
         torch.cuda.synchronize()
         res["t_train"] = time.perf_counter() - t0
         del T

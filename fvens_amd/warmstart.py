@@ -56,23 +56,32 @@ def cosine_lr(step: int, total: int, lr0: float, lr_min: float, warmup: int = 50
     return lr_min + 0.5 * (lr0 - lr_min) * (1 + np.cos(np.pi * min(1.0, t)))
 
 
+def usable(stats, max_rel: float = 1e-2):
+    """Indices of training fields worth learning: converged solves, and solves that stopped at
+    the step limit within max_rel of their normaliser; non-finite solves are dropped."""
+    return [i for i, s in enumerate(stats)
+            if s["status"] == 0 or (s["status"] == 4 and s["final_rel"] < max_rel)]
+
+
 def train(net: "fvx.MPNN", targets, mach, aoa_deg, steps: int, graphs_per_step: int,
           lr0: float = 2e-3, lr_min: float = 5e-5, seed: int = 0, log_every: int = 0,
-          log=print):
+          log=print, valid=None):
     """Adam training on a resident target set [nsamples, ncell, 4] whose sample i was solved
-    at (mach[i], aoa_deg[i]); each step draws `graphs_per_step` samples without replacement."""
+    at (mach[i], aoa_deg[i]); each step draws `graphs_per_step` samples without replacement
+    from `valid` (default: all samples)."""
     rng = np.random.default_rng(seed)
     mach = np.asarray(mach)
     aoa_deg = np.asarray(aoa_deg)
-    ns = len(mach)
+    pool = np.arange(len(mach)) if valid is None else np.asarray(valid)
+    ns = len(pool)
     G = min(graphs_per_step, ns, net.max_graphs)
     history = []
-    order = rng.permutation(ns)
+    order = pool[rng.permutation(ns)]
     pos = 0
     t0 = time.perf_counter()
     for it in range(steps):
         if pos + G > ns:
-            order, pos = rng.permutation(ns), 0
+            order, pos = pool[rng.permutation(ns)], 0
         ids = order[pos:pos + G]
         pos += G
         net.set_points(mach[ids], aoa_deg[ids])
