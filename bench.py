@@ -148,6 +148,56 @@ def gpu_time_to_tol(name, tol, precision, sweeps=16):
                 CL=f["CL"], CDp=f["CDp"], precision=precision, pc=f"multicolour SGS x{sweeps}")
 
 
+def viscous_time_to_tol(precision, tol=1e-10, oracle=False):
+    """The reference's laminar NACA0012 case (testcases/visc-naca0012/laminar-implicit.ctrl,
+    13,156-cell hybrid mesh, Roe + least squares, adiabatic wall, M 0.5, Re 5000): first-order
+    starter (CFL 200 -> 1000 to 1e-1) then the main solve (CFL 500 -> 5000) to tol x the first
+    main-step norm. oracle=True times the CPU oracle (natural-order ILU(0), one thread)."""
+    import numpy as np
+    from fvens_amd import gmsh
+    mesh_file = os.path.join(ROOT, "tests", "golden", "NACA0012_lam_hybrid_1.msh")
+    raw = gmsh.read_gmsh2(mesh_file)
+    bcs = [(4, "inflowoutflow"), (2, "adiabaticwall", (0.0, 0.0))]
+    phys = dict(Minf=0.5, aoa_deg=0.0, Tinf=288.15, Reinf=5000.0, Pr=0.72, viscous=True, bcs=bcs)
+    if oracle:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        import pyoracle as po
+        m = po.Mesh(raw)
+        d1 = po.Disc(m, po.make_config(**phys, flux="ROE", gradient="LEASTSQUARES", order2=False))
+        d2 = po.Disc(m, po.make_config(**phys, flux="ROE", gradient="LEASTSQUARES", recon="NONE"))
+        u = np.tile(d1.freestream(), (raw.nelem, 1))
+        t0 = time.perf_counter()
+        u, s1 = d1.steady_solve(u, cflinit=200.0, cflfin=1000.0, tol=1e-1, maxiter=50,
+                                fullupdate=1, min_relax=0.2, pc="ilu0")
+        u, s2 = d2.steady_solve(u, cflinit=500.0, cflfin=5000.0, tol=tol, maxiter=400,
+                                fullupdate=1, min_relax=0.2, pc="ilu0")
+        f = d2.functionals(u, 2)
+        return dict(seconds=round(time.perf_counter() - t0, 3), steps=s1["steps"] + s2["steps"],
+                    converged=bool(s2["converged"]), CL=f["CL"], CDp=f["CDp"], CDsf=f["CDsf"],
+                    threads=1, pc="natural-order block ILU(0)")
+    import torch
+    from fvens_amd import fvx
+    gm = fvx.Mesh(raw)
+    d1 = fvx.FlowFV(gm, fvx.physics_cfg(**phys),
+                    fvx.numerics_cfg(flux="ROE", gradient="LEASTSQUARES", order2=False))
+    d2 = fvx.FlowFV(gm, fvx.physics_cfg(**phys),
+                    fvx.numerics_cfg(flux="ROE", gradient="LEASTSQUARES", recon="NONE"))
+    u = torch.tensor(np.tile(d1.freestream(), (raw.nelem, 1)), device="cuda")
+    kw = dict(pc="sgs", pc_sweeps=16, pc_precision=precision, jac_precision=precision,
+              min_relax=0.2, fullupdate=True, raise_on_failure=False)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    s1 = d1.steady_solve(u, cflinit=200.0, cflfin=1000.0, tol=1e-1, maxiter=50, **kw)
+    s2 = d2.steady_solve(u, cflinit=500.0, cflfin=5000.0, tol=tol, maxiter=400, **kw)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    f = d2.functionals(u, 2)
+    return dict(seconds=round(t, 4), starter_steps=s1["steps"], steps=s2["steps"],
+                lin_iters=s2["lin_iters"], final_rel=s2["final_rel"],
+                converged=bool(s2["converged"]), CL=f["CL"], CDp=f["CDp"], CDsf=f["CDsf"],
+                precision=precision, pc="multicolour SGS x16")
+
+
 def cpu_time_to_tol(name, tol):
     """The oracle's FP64 implicit solve of the same case: natural-order block ILU(0), the
     reference's bjacobi + ILU on one rank (tests/inv-gaussianbump/ibump.solverc)."""
@@ -419,6 +469,20 @@ def main():
             tt["bump_1e-10"]["cpu_oracle_fp64"] = cpu_time_to_tol("bump", 1e-10)
             tt["bump_1e-10"]["gpu_vs_cpu"] = round(
                 tt["bump_1e-10"]["cpu_oracle_fp64"]["seconds"] / bump64["seconds"], 2)
+        v64 = viscous_time_to_tol("f64")
+        v32 = viscous_time_to_tol("f32")
+        tt["viscous_naca0012_1e-10"] = {
+            "case": "the reference's laminar NACA0012 case (13,156-cell hybrid mesh, Roe, least "
+                    "squares, adiabatic wall, M0.5, Re 5000), starter then main solve to 1e-10",
+            "gpu_fp64": v64, "gpu_fp32_jacobian_and_pc": v32,
+            "fp32_speedup": round(v64["seconds"] / v32["seconds"], 3),
+            "CL_CDp_CDsf_fp32_vs_fp64": [abs(v32["CL"] - v64["CL"]),
+                                         abs(v32["CDp"] / v64["CDp"] - 1),
+                                         abs(v32["CDsf"] / v64["CDsf"] - 1)]}
+        if not a.no_cpu_baseline:
+            vo = viscous_time_to_tol("f64", oracle=True)
+            tt["viscous_naca0012_1e-10"]["cpu_oracle_fp64"] = vo
+            tt["viscous_naca0012_1e-10"]["gpu_vs_cpu"] = round(vo["seconds"] / v64["seconds"], 2)
         if os.path.exists(a.warm_params):
             tt["warm_start_airfoil"] = warm_start_ab(a.warm_params, a.warm_points, 4,
                                                      a.batch_maxiter)
