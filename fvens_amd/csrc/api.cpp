@@ -357,6 +357,15 @@ fvx_status fvx_batch_steady_solve(fvx_batch b, const fvx_solver_cfg* cfg, double
 	});
 }
 
+fvx_status fvx_solve_history(fvx_disc d, fvx_step_record* out, int cap, int* n)
+{
+	return guarded([&] {
+		require(d && n && (out || cap == 0), "null argument");
+		require(cap >= 0, "negative capacity");
+		fvx::solve_history(*d, out, cap, n);
+	});
+}
+
 fvx_status fvx_batch_set_norms(fvx_batch b, const double* norms)
 {
 	return guarded([&] {

@@ -270,6 +270,8 @@ void steady_core(fvx_disc_s& d, const fvx_solver_cfg& c, double* u, int ninst,
                  const GasParams* gp, std::vector<fvx_solve_stats>& stats, hipStream_t s,
                  const double* norms = nullptr);
 
+void solve_history(const fvx_disc_s& d, fvx_step_record* out, int cap, int* n);
+
 // autotuner (tune.cpp)
 void autotune_launch(fvx_disc_s& d, const double* u, int reps, std::vector<fvx_tune_row>& rows,
                      hipStream_t s);

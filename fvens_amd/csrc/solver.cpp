@@ -376,8 +376,16 @@ struct SolverWork {
 	DBuf<double> r, dt, dtm, du, aux, raux, grad, rgrad, cfl;
 	double* cfl_pin = nullptr;
 	Krylov kry;
+	std::vector<fvx_step_record> hist;     // per-step record of the last one-instance solve
 	~SolverWork() { if(cfl_pin) (void)hipHostFree(cfl_pin); }
 };
+
+void solve_history(const fvx_disc_s& d, fvx_step_record* out, int cap, int* n)
+{
+	const int k = d.work ? (int)d.work->hist.size() : 0;
+	*n = k;
+	for(int i = 0; i < k && i < cap; i++) out[i] = d.work->hist[i];
+}
 
 void destroy_solver_work(SolverWork* w) { delete w; }
 
@@ -458,6 +466,9 @@ void steady_core(fvx_disc_s& d, const fvx_solver_cfg& c, double* u, int ninst,
 	auto now = [&] { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
 	std::vector<double> t_end(ninst, 0.0);
 	std::vector<int> its;
+	// the reference's SteadyStepMonitor (src/ode/aodesolver.cpp:536-549), one-instance solves
+	w.hist.clear();
+	double linw = 0.0;
 	for(;;) {
 		std::vector<int> act;
 		for(int i = 0; i < ninst; i++) if(active(i)) act.push_back(i);
@@ -516,9 +527,11 @@ void steady_core(fvx_disc_s& d, const fvx_solver_cfg& c, double* u, int ninst,
 				};
 			std::vector<double> rn2(ninst);
 			for(int i : act) rn2[i] = K.h(i, Krylov::RES);
+			const double tl0 = now();
 			K.solve(op, [&](const double* r, long sr, double* z, long sz, const Inst& LL) {
 			            launch_pc_apply(w.P, r, sr, z, sz, LL, s); },
 			        w.r.p, w.du.p, c.krylov_restart, c.krylov_rtol, act, single, its, s);
+			linw += now() - tl0;
 			std::vector<int> upd;
 			for(int i : act) {
 				if(its[i] < 0) { stats[i].status = FVX_ERR_NONFINITE; t_end[i] = now(); continue; }
@@ -550,6 +563,10 @@ void steady_core(fvx_disc_s& d, const fvx_solver_cfg& c, double* u, int ninst,
 			else {
 				if(is[i].step == 0 && !norms && c.fixed_norm <= 0) is[i].initres = is[i].resi;
 				is[i].step++;
+				if(ninst == 1)
+					w.hist.push_back(fvx_step_record{is[i].step, is[i].resi/is[i].initres,
+					                                 is[i].resi, now(), linw,
+					                                 stats[i].lin_iters, is[i].cfl});
 			}
 			if(!active(i)) t_end[i] = now();
 		}

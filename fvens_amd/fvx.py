@@ -301,6 +301,21 @@ class FlowFV:
             check(rc)
         return out
 
+    def solve_history(self):
+        """Per-step records (step, rel, abs, walltime, lin_walltime, lin_iters, cfl) of the last
+        one-instance steady solve on this discretisation."""
+        n = C.c_int()
+        check(lib().fvx_solve_history(self.h, None, 0, C.byref(n)))
+        recs = (StepRecord * max(1, n.value))()
+        check(lib().fvx_solve_history(self.h, recs, n.value, C.byref(n)))
+        return [{f: getattr(r, f) for f, _ in StepRecord._fields_} for r in recs[:n.value]]
+
+
+class StepRecord(C.Structure):
+    _fields_ = [("step", C.c_int), ("rel", C.c_double), ("abs", C.c_double),
+                ("walltime", C.c_double), ("lin_walltime", C.c_double), ("lin_iters", C.c_int),
+                ("cfl", C.c_double)]
+
 
 PREC = {"f64": 0, "f32": 1}
 

@@ -1,0 +1,295 @@
+/* control_file.hpp -- reader of the reference's flow control files.
+ *
+ * The files are in the INFO format of Boost.PropertyTree (pt::read_info): "key value" lines,
+ * values optionally in double quotes, "key { ... }" sections whose brace may open on the next
+ * line, and comments from ';' to the end of the line. parse_flow_controlfile restates the
+ * reference's parse_flow_controlfile (src/utilities/controlparser.cpp:60-216, boundary
+ * conditions :248-291, boundary names src/spatial/abctypemap.cpp:15-31) for the steady subset:
+ * the same keys, defaults and upper-casing of option strings. Differences, both deliberate:
+ * spatial_discretization.limiter_parameter is read (the reference never parses it and leaves
+ * the parameter uninitialised), and unsteady or RANS settings are rejected.
+ */
+#ifndef FVENS_GPU_CONTROL_FILE_HPP
+#define FVENS_GPU_CONTROL_FILE_HPP
+
+#include "fvens_gpu.hpp"
+#include <algorithm>
+#include <cctype>
+#include <cmath>
+#include <fstream>
+#include <sstream>
+#include <utility>
+
+namespace fvens_gpu {
+
+/// A node of an INFO tree: its value and its children in file order.
+struct InfoNode {
+	std::string value;
+	std::vector<std::pair<std::string, InfoNode>> children;
+
+	const InfoNode* child(const std::string& key) const
+	{
+		for(const auto& c : children) if(c.first == key) return &c.second;
+		return nullptr;
+	}
+	/// Node at a dotted path ("a.b.c"), or null.
+	const InfoNode* find(const std::string& path) const
+	{
+		const InfoNode* n = this;
+		size_t b = 0;
+		while(n && b <= path.size()) {
+			const size_t e = path.find('.', b);
+			n = n->child(path.substr(b, e == std::string::npos ? std::string::npos : e - b));
+			if(e == std::string::npos) break;
+			b = e + 1;
+		}
+		return n;
+	}
+	std::string get(const std::string& path) const
+	{
+		const InfoNode* n = find(path);
+		if(!n) throw std::runtime_error("control file: missing entry " + path);
+		return n->value;
+	}
+	std::string get(const std::string& path, const std::string& dflt) const
+	{
+		const InfoNode* n = find(path);
+		return n ? n->value : dflt;
+	}
+};
+
+namespace detail {
+
+/// Tokens of an INFO file: words (quoted strings unquoted), "{", "}" and end-of-line markers.
+inline std::vector<std::string> info_tokens(std::istream& in)
+{
+	std::vector<std::string> t;
+	std::string line;
+	while(std::getline(in, line)) {
+		size_t i = 0;
+		while(i < line.size()) {
+			const char c = line[i];
+			if(c == ';') break;
+			if(std::isspace((unsigned char)c)) { i++; continue; }
+			if(c == '{' || c == '}') { t.push_back(std::string(1, c)); i++; continue; }
+			std::string w;
+			if(c == '"') {
+				i++;
+				while(i < line.size() && line[i] != '"') {
+					if(line[i] == '\\' && i + 1 < line.size()) i++;
+					w += line[i++];
+				}
+				i++;
+				t.push_back("\"" + w);            // marks a quoted value
+				continue;
+			}
+			while(i < line.size() && !std::isspace((unsigned char)line[i]) && line[i] != '{'
+			      && line[i] != '}' && line[i] != ';')
+				w += line[i++];
+			t.push_back(w);
+		}
+		t.push_back("\n");
+	}
+	return t;
+}
+
+inline std::string info_word(const std::string& tok)
+{
+	return !tok.empty() && tok[0] == '"' ? tok.substr(1) : tok;
+}
+
+inline size_t info_parse(const std::vector<std::string>& t, size_t i, InfoNode& node, bool top)
+{
+	while(i < t.size()) {
+		if(t[i] == "\n") { i++; continue; }
+		if(t[i] == "}") {
+			if(top) throw std::runtime_error("control file: unmatched '}'");
+			return i + 1;
+		}
+		if(t[i] == "{") throw std::runtime_error("control file: '{' without a key");
+		std::pair<std::string, InfoNode> kv;
+		kv.first = info_word(t[i++]);
+		// the value: the rest of the line up to a '{'
+		std::string v;
+		while(i < t.size() && t[i] != "\n" && t[i] != "{" && t[i] != "}") {
+			if(!v.empty()) v += ' ';
+			v += info_word(t[i++]);
+		}
+		kv.second.value = v;
+		// a section opens on this line or on the next non-empty one
+		size_t j = i;
+		while(j < t.size() && t[j] == "\n") j++;
+		if(j < t.size() && t[j] == "{") i = info_parse(t, j + 1, kv.second, false);
+		node.children.push_back(std::move(kv));
+	}
+	if(!top) throw std::runtime_error("control file: missing '}'");
+	return i;
+}
+
+inline std::string upper(std::string s)
+{
+	for(char& c : s) c = (char)std::toupper((unsigned char)c);
+	return s;
+}
+inline std::string lower(std::string s)
+{
+	for(char& c : s) c = (char)std::tolower((unsigned char)c);
+	return s;
+}
+template <class T> std::vector<T> to_vector(const std::string& s)
+{
+	std::istringstream is(s);
+	std::vector<T> v;
+	T x;
+	while(is >> x) v.push_back(x);
+	return v;
+}
+inline bool to_bool(const std::string& s)
+{
+	const std::string l = lower(s);
+	if(l == "true" || l == "1") return true;
+	if(l == "false" || l == "0") return false;
+	throw std::runtime_error("control file: not a boolean: " + s);
+}
+}
+
+inline InfoNode read_info(std::istream& in)
+{
+	const std::vector<std::string> t = detail::info_tokens(in);
+	InfoNode root;
+	detail::info_parse(t, 0, root, true);
+	return root;
+}
+
+inline InfoNode read_info_file(const std::string& path)
+{
+	std::ifstream f(path);
+	if(!f) throw std::runtime_error("cannot open control file " + path);
+	return read_info(f);
+}
+
+/// The steady-flow subset of the reference's FlowParserOptions (controlparser.hpp:19-70).
+struct FlowParserOptions {
+	std::string meshfile, vtu_output_file, logfile, flowtype, invflux, invfluxjac,
+		gradientmethod, limiter, pseudotimetype, surfnameprefix, sim_type, nl_update_scheme;
+	double initcfl = 0, endcfl = 0, tolerance = 0, firstinitcfl = 0, firstendcfl = 0,
+		firsttolerance = 0, Minf = 0, alpha = 0, Reinf = 0, Tinf = 0, Pr = 0, gamma = 1.4,
+		limiter_param = 0, min_nl_update = 0.2;
+	int maxiter = 0, firstmaxiter = 0;
+	std::vector<FlowBCConfig> bcconf;
+	bool usestarter = false, lognres = false, useconstvisc = false, viscsim = false,
+		order2 = true;
+	std::vector<int> lwalls, lothers;
+};
+
+inline BCType bc_type_from_name(const std::string& s)
+{
+	static const std::pair<const char*, BCType> names[] = {
+		{"slipwall", SLIP_WALL_BC}, {"isothermalwall", ISOTHERMAL_WALL_BC},
+		{"adiabaticwall", ADIABATIC_WALL_BC}, {"farfield", FARFIELD_BC},
+		{"inflowoutflow", INFLOW_OUTFLOW_BC}, {"subsonic_inflow", SUBSONIC_INFLOW_BC},
+		{"extrapolation", EXTRAPOLATION_BC}, {"periodic", PERIODIC_BC}};
+	for(const auto& n : names) if(s == n.first) return n.second;
+	throw std::runtime_error("control file: unknown boundary condition type " + s);
+}
+
+inline FlowParserOptions parse_flow_controlfile(const InfoNode& in)
+{
+	using detail::upper;
+	FlowParserOptions o;
+	o.meshfile = in.get("io.mesh_file");
+	o.vtu_output_file = in.get("io.solution_output_file");
+	o.logfile = in.get("io.log_file_prefix");
+	o.lognres = detail::to_bool(in.get("io.convergence_history_required"));
+
+	o.flowtype = upper(in.get("flow_conditions.flow_type"));
+	o.gamma = std::stod(in.get("flow_conditions.adiabatic_index"));
+	o.alpha = M_PI/180.0*std::stod(in.get("flow_conditions.angle_of_attack"));
+	o.Minf = std::stod(in.get("flow_conditions.freestream_Mach_number"));
+	if(o.flowtype == "NAVIERSTOKES" || o.flowtype == "RANS") {
+		if(o.flowtype == "RANS") throw std::runtime_error("control file: flow type RANS is not supported");
+		o.viscsim = true;
+		o.Tinf = std::stod(in.get("flow_conditions.freestream_temperature"));
+		o.Reinf = std::stod(in.get("flow_conditions.freestream_Reynolds_number"));
+		o.Pr = std::stod(in.get("flow_conditions.Prandtl_number"));
+		o.useconstvisc = detail::to_bool(in.get("flow_conditions.use_constant_viscosity",
+		                                        "false"));
+	} else {
+		o.viscsim = false;
+		o.Tinf = 298.0;
+		o.Reinf = INFINITY;
+		o.Pr = NAN;
+	}
+
+	for(int ibc = 0; ; ibc++) {
+		const std::string key = "bc.bc" + std::to_string(ibc);
+		if(!in.find(key)) break;
+		FlowBCConfig b;
+		b.bc_type = bc_type_from_name(detail::lower(in.get(key + ".type")));
+		b.bc_tag = std::stoi(in.get(key + ".marker"));
+		if(b.bc_type == ADIABATIC_WALL_BC || b.bc_type == ISOTHERMAL_WALL_BC
+		   || b.bc_type == SUBSONIC_INFLOW_BC)
+			b.bc_vals = detail::to_vector<double>(in.get(key + ".boundary_values"));
+		if(b.bc_type == PERIODIC_BC)
+			b.bc_opts = detail::to_vector<int>(in.get(key + ".options"));
+		o.bcconf.push_back(b);
+	}
+	o.lwalls = detail::to_vector<int>(in.get("bc.listof_output_wall_boundaries", ""));
+	o.lothers = detail::to_vector<int>(in.get("bc.listof_output_other_boundaries", ""));
+	if(!o.lwalls.empty() || !o.lothers.empty())
+		o.surfnameprefix = in.get("bc.surface_output_file_prefix");
+
+	o.sim_type = upper(in.get("time.simulation_type"));
+	if(o.sim_type != "STEADY")
+		throw std::runtime_error("control file: only steady simulations are supported");
+
+	o.invflux = upper(in.get("spatial_discretization.inviscid_flux"));
+	o.gradientmethod = upper(in.get("spatial_discretization.gradient_method"));
+	if(o.gradientmethod == "NONE") o.order2 = false;
+	o.limiter = upper(in.get("spatial_discretization.limiter"));
+	o.limiter_param = std::stod(in.get("spatial_discretization.limiter_parameter", "20.0"));
+
+	o.pseudotimetype = upper(in.get("pseudotime.pseudotime_stepping_type"));
+	o.initcfl = std::stod(in.get("pseudotime.main.cfl_min"));
+	o.endcfl = std::stod(in.get("pseudotime.main.cfl_max"));
+	o.tolerance = std::stod(in.get("pseudotime.main.tolerance"));
+	o.maxiter = std::stoi(in.get("pseudotime.main.max_timesteps"));
+	if(in.find("pseudotime.initialization")) {
+		o.usestarter = true;
+		o.firstinitcfl = std::stod(in.get("pseudotime.initialization.cfl_min"));
+		o.firstendcfl = std::stod(in.get("pseudotime.initialization.cfl_max"));
+		o.firsttolerance = std::stod(in.get("pseudotime.initialization.tolerance"));
+		o.firstmaxiter = std::stoi(in.get("pseudotime.initialization.max_timesteps"));
+	}
+	if(o.pseudotimetype == "IMPLICIT") {
+		o.invfluxjac = upper(in.get("Jacobian_inviscid_flux"));
+		if(o.invfluxjac == "CONSISTENT") o.invfluxjac = o.invflux;
+		o.nl_update_scheme = upper(in.get("pseudotime.nonlinear_update_scheme"));
+		o.min_nl_update = std::stod(in.get("pseudotime.min_nonlinear_relaxation_factor", "0.2"));
+	}
+	return o;
+}
+
+/// extract_spatial_physics_config (src/utilities/controlparser.cpp:218-225)
+inline FlowPhysicsConfig extract_spatial_physics_config(const FlowParserOptions& o)
+{
+	return FlowPhysicsConfig{o.gamma, o.Minf, o.Tinf, o.Reinf, o.Pr, o.alpha, o.viscsim,
+	                         o.useconstvisc, o.bcconf};
+}
+
+/// extract_spatial_numerics_config (src/utilities/controlparser.cpp:227-232)
+inline FlowNumericsConfig extract_spatial_numerics_config(const FlowParserOptions& o)
+{
+	return FlowNumericsConfig{o.invflux, o.invfluxjac.empty() ? o.invflux : o.invfluxjac,
+	                          o.gradientmethod, o.limiter, o.limiter_param, o.order2};
+}
+
+/// firstorder_spatial_numerics_config (src/utilities/controlparser.cpp:234-239)
+inline FlowNumericsConfig firstorder_spatial_numerics_config(const FlowParserOptions& o)
+{
+	return FlowNumericsConfig{o.invflux, o.invfluxjac.empty() ? o.invflux : o.invfluxjac,
+	                          "NONE", "NONE", 1.0, false};
+}
+
+}
+#endif

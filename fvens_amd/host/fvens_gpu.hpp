@@ -163,6 +163,13 @@ public:
 	int gninface() const { return ninface_; }
 	int gnaface() const { return naface_; }
 	fvx_mesh handle() const { return h_; }
+	/// Cell areas (UMesh::garea, src/mesh/mesh.cpp:290-313).
+	std::vector<double> areas() const
+	{
+		std::vector<double> a(nelem_);
+		check(fvx_mesh_faces(h_, nullptr, nullptr, a.data(), nullptr));
+		return a;
+	}
 private:
 	fvx_mesh h_ = nullptr;
 	int npoin_, nelem_, nbface_, ninface_, naface_;
@@ -248,6 +255,16 @@ public:
 		double out[4];
 		check(fvx_functionals(h_, u, wall_marker, out, stream));
 		return FlowSolutionFunctionals{ 0.0, out[3], out[0], out[1], out[2] };
+	}
+	/// Step records of the last steady solve on this discretisation (TimingData::convhis,
+	/// src/ode/aodesolver.hpp:46-67).
+	std::vector<fvx_step_record> solve_history() const
+	{
+		int n = 0;
+		check(fvx_solve_history(h_, nullptr, 0, &n));
+		std::vector<fvx_step_record> h(n);
+		if(n) check(fvx_solve_history(h_, h.data(), n, &n));
+		return h;
 	}
 	const UMesh* mesh() const { return m_; }
 	fvx_disc handle() const { return h_; }

@@ -236,6 +236,20 @@ fvx_status fvx_batch_freestream(fvx_batch b, double* out);
 /* fvx_compute_residual for every instance (u, r: [ninst][ncell][4]; dt: [ninst][ncell]). */
 fvx_status fvx_batch_compute_residual(fvx_batch b, const double* u, double* r, int gettimesteps,
                                       double* dt, void* stream);
+/* One pseudo-time step of a solve: the reference's SteadyStepMonitor
+ * (src/ode/aodesolver.hpp:35-43, filled at src/ode/aodesolver.cpp:536-549): step number (from
+ * 1), relative and absolute residual norm at the start of the step, elapsed wall time and
+ * cumulative linear-solver wall time (seconds), cumulative linear iterations, CFL number. */
+typedef struct {
+	int step;
+	double rel, abs, walltime, lin_walltime;
+	int lin_iters;
+	double cfl;
+} fvx_step_record;
+/* Step records of the last steady solve on d that had one instance (host out[cap]); *n receives
+ * the number of steps (records beyond cap are not written). */
+fvx_status fvx_solve_history(fvx_disc d, fvx_step_record* out, int cap, int* n);
+
 /* fvx_steady_solve for every instance; stats[ninst] receives each instance's outcome in its
  * status field (FVX_OK, FVX_ERR_NONFINITE or FVX_ERR_NOT_CONVERGED); a failing instance does
  * not stop the others, and the call itself returns FVX_OK unless an argument or the device
