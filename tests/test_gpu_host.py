@@ -62,6 +62,7 @@ def test_cpp_case_driver_runs_the_reference_control_file(torch_cuda, tmp_path):
     rows = [ln.split() for ln in hist if not ln.startswith("#")]
     assert len(rows) >= 10
     assert [int(r[0]) for r in rows] == list(range(1, len(rows) + 1))
-    # log10 of the relative residual reaches the file's tolerance, 1e-6, at the last step
-    assert float(rows[0][1]) == 0.0 and float(rows[-1][1]) > -6.01
-    assert min(float(r[1]) for r in rows) < -5.0
+    # log10 of the relative residual starts at 0 and the last recorded step is the one whose
+    # residual met the file's tolerance, 1e-6
+    assert float(rows[0][1]) == 0.0 and float(rows[-1][1]) <= -5.999
+    assert all(float(r[1]) > -6.0 for r in rows[:-1])
