@@ -537,6 +537,28 @@ fvx_status fvx_functionals(fvx_disc d, const double* u, int wall_tag, double* ou
 	});
 }
 
+fvx_status fvx_surface_data(fvx_disc d, const double* u, int wall_tag, double* out, int cap,
+                            int* nfaces, void* stream)
+{
+	return guarded([&] {
+		require(d && u && nfaces && (out || cap == 0), "null argument");
+		const fvx_mesh_s& m = *d->m;
+		std::vector<int> list;
+		for(int b = 0; b < m.nbface; b++) if(m.btag[b] == wall_tag) list.push_back(b);
+		*nfaces = (int)list.size();
+		if(!out || list.empty()) return;
+		require(cap >= (int)list.size(), "output holds fewer rows than the marker has faces");
+		hipStream_t s = static_cast<hipStream_t>(stream);
+		fvx::DBuf<int> dl;
+		fvx::DBuf<double> dout;
+		dl.upload(list);
+		dout.alloc(4*list.size());
+		fvx::launch_gradients(*d, u, d->d_grad.p, nullptr, 1, s);
+		fvx::launch_surface_data(*d, u, d->d_grad.p, dl.p, (int)list.size(), dout.p, s);
+		FVX_HIP(hipMemcpyAsync(out, dout.p, 4*list.size()*sizeof(double), hipMemcpyDeviceToHost, s));
+		FVX_HIP(hipStreamSynchronize(s));
+	});
+}
 
 // ---- learned warm start ----
 

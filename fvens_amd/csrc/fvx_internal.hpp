@@ -271,6 +271,8 @@ void steady_core(fvx_disc_s& d, const fvx_solver_cfg& c, double* u, int ninst,
                  const double* norms = nullptr);
 
 void solve_history(const fvx_disc_s& d, fvx_step_record* out, int cap, int* n);
+void launch_surface_data(const fvx_disc_s& d, const double* u, const double* cgrad,
+                         const int* list, int nw, double* out, hipStream_t s);
 
 // autotuner (tune.cpp)
 void autotune_launch(fvx_disc_s& d, const double* u, int reps, std::vector<fvx_tune_row>& rows,

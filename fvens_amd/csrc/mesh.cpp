@@ -339,8 +339,9 @@ void build_disc(fvx_disc_s& d)
 		throw Error(FVX_ERR_INVALID, "unknown inviscid flux");
 	if(n.gradientscheme < 0 || n.gradientscheme > 2 || n.reconstruction < 0 || n.reconstruction > 4)
 		throw Error(FVX_ERR_INVALID, "unknown gradient scheme or reconstruction");
-	if(n.reconstruction == FVX_RECON_WENO)
-		throw Error(FVX_ERR_UNSUPPORTED, "WENO reconstruction is not built on the device");
+	if(n.reconstruction == FVX_RECON_WENO && !(n.limiter_param >= 0.0 && std::isfinite(n.limiter_param)))
+		throw Error(FVX_ERR_INVALID, "WENO reconstruction needs a finite, non-negative central "
+		            "weight (limiter parameter)");
 
 	const int ne = m.nelem;
 	std::vector<double> coef((size_t)ne*8, 0.0), eps2(ne, 0.0);

@@ -142,6 +142,56 @@ void k_gradients(int ncell, const double* __restrict__ u, const int* __restrict_
 	}
 }
 
+/// WENO-weighted gradients (WENOReconstruction::compute_face_values,
+/// src/spatial/limitedlinearreconstruction.cpp:39-98): per cell and primitive variable v,
+/// g' = (lambda g_i / (|g_i|^2 + eps)^4 + sum_j g_j / (|g_j|^2 + eps)^4) / (sum of the weights)
+/// over the cell's non-boundary neighbours j, eps = 1e-5. Face values are then the linear
+/// extrapolation with g', which is the flux pass's limited-gradient input. Four lanes per cell,
+/// one per variable; each lane reads the 16-byte x/y pair of its variable from every gradient.
+__global__ __launch_bounds__(MAX_BLOCK)
+void k_weno(int ncell, const double* __restrict__ grad, const int* __restrict__ nbr,
+            double lambda, const BatchArg ba, double* __restrict__ rout)
+{
+	const long I = ba.list ? ba.list[blockIdx.y] : 0;
+	grad += I*8L*ncell;
+	rout += I*8L*ncell;
+	const int v = threadIdx.x & 3;
+	const int cell = blockIdx.x*(int)(blockDim.x >> 2) + (threadIdx.x >> 2);
+	if(cell >= ncell) return;
+	const int4 nb = reinterpret_cast<const int4*>(nbr)[cell];
+	const int nbs[4] = { nb.x, nb.y, nb.z, nb.w };
+	double2 gj[4];
+#pragma unroll
+	for(int k = 0; k < 4; k++)
+		gj[k] = reinterpret_cast<const double2*>(grad)[4*(nbs[k] >= 0 ? nbs[k] : cell) + v];
+	const double2 gi = reinterpret_cast<const double2*>(grad)[4*cell + v];
+	const double eps = 1.0e-5;
+	double m = gi.x*gi.x + gi.y*gi.y + eps;
+	m *= m;
+	double w = lambda/(m*m);
+	double wsum = w, lx = w*gi.x, ly = w*gi.y;
+#pragma unroll
+	for(int k = 0; k < 4; k++) {
+		if(nbs[k] < 0) continue;            // boundary (ghost) or unused slot
+		double mj = gj[k].x*gj[k].x + gj[k].y*gj[k].y + eps;
+		mj *= mj;
+		const double wj = 1.0/(mj*mj);
+		wsum += wj;
+		lx += wj*gj[k].x;
+		ly += wj*gj[k].y;
+	}
+	reinterpret_cast<double2*>(rout)[4*cell + v] = make_double2(lx/wsum, ly/wsum);
+}
+
+void launch_weno(const fvx_disc_s& d, const double* grad, double* rgrad, hipStream_t s,
+                 const BatchArg& ba, int nact)
+{
+	const int ncell = d.m->nelem, block = 256;
+	hipLaunchKernelGGL(k_weno, dim3((ncell + block/4 - 1)/(block/4), nact), dim3(block), 0, s,
+	                   ncell, grad, d.m->d_nbr.p, d.nc.limiter_param, ba, rgrad);
+	FVX_HIP(hipGetLastError());
+}
+
 /* ------------------------------------------------------------------ profiler ----------- */
 
 void Profiler::start(int k, hipStream_t s)
@@ -229,6 +279,10 @@ void launch_residual(const fvx_disc_s& d, const double* u, double* r, int gettim
 		if(limited) {
 			launch_gradients(d, u, visc ? grad : nullptr, rgrad, 0, s, ba, nact);
 			rg = rgrad; ug = grad; mode = 1;
+		} else if(rec == FVX_RECON_WENO) {
+			launch_gradients(d, u, grad, nullptr, 0, s, ba, nact);
+			launch_weno(d, grad, rgrad, s, ba, nact);
+			rg = rgrad; ug = grad; mode = 1;
 		} else {
 			launch_gradients(d, u, grad, nullptr, 0, s, ba, nact);
 			rg = grad; ug = grad;
@@ -273,7 +327,8 @@ void k_functionals(int ncell, int nbface, const double* __restrict__ u,
 		double w[4];
 		load4(u + 4*c, w);
 		const double cp = (pressure(gp, w) - gp.pinf)*2.0;
-		const double mu = viscosity(gp, w);
+		// Sutherland's law also with constant viscosity (flow_spatial.cpp:226)
+		const double mu = sutherland(gp, pressure(gp, w)/w[0]*gp.gM2);
 		double gu[2][2];
 		for(int i = 0; i < 2; i++)
 			for(int j = 0; j < 2; j++)
@@ -316,5 +371,44 @@ void launch_functionals(const fvx_disc_s& d, const double* u, const double* cgra
 }
 
 int functional_blocks() { return FN_BLOCKS; }
+
+/// Per wall face of `list` (boundary-face indices in boundary order): face centre, Cp and Cf from
+/// the cell-centred state and conserved gradients of the host cell -- the output matrix of
+/// computeSurfaceData (src/spatial/flow_spatial.cpp:130-310), rows [nw][4].
+__global__ __launch_bounds__(BLOCK)
+void k_surface_data(int nw, const int* __restrict__ list, const double* __restrict__ u,
+                    const double* __restrict__ cgrad, const int* __restrict__ bhost,
+                    const FaceGeom* __restrict__ fgeom, GasParams gp, double* __restrict__ out)
+{
+	const int i = blockIdx.x*BLOCK + threadIdx.x;
+	if(i >= nw) return;
+	const int f = list[i];
+	const int c = bhost[f];
+	const FaceGeom G = fgeom[f];
+	double w[4];
+	load4(u + 4*c, w);
+	const double cp = (pressure(gp, w) - gp.pinf)*2.0;
+	const double mu = sutherland(gp, pressure(gp, w)/w[0]*gp.gM2);
+	double gu[2][2];
+	for(int a = 0; a < 2; a++)
+		for(int b = 0; b < 2; b++)
+			gu[a][b] = (cgrad[8*c + (a+1)*2 + b]*w[0] - w[a+1]*cgrad[8*c + b])/(w[0]*w[0]);
+	const double n[2] = { G.nx, G.ny }, t[2] = { G.ny, -G.nx };
+	double fo[2];
+	for(int a = 0; a < 2; a++) fo[a] = (gu[a][0] + gu[0][a])*n[0] + (gu[a][1] + gu[1][a])*n[1];
+	out[4*i] = G.mx;
+	out[4*i + 1] = G.my;
+	out[4*i + 2] = cp;
+	out[4*i + 3] = 2*mu*(fo[0]*t[0] + fo[1]*t[1]);
+}
+
+void launch_surface_data(const fvx_disc_s& d, const double* u, const double* cgrad,
+                         const int* list, int nw, double* out, hipStream_t s)
+{
+	if(nw == 0) return;
+	hipLaunchKernelGGL(k_surface_data, dim3((nw + BLOCK - 1)/BLOCK), dim3(BLOCK), 0, s, nw, list,
+	                   u, cgrad, d.m->d_bhost.p, d.m->d_fgeom.p, d.gas, out);
+	FVX_HIP(hipGetLastError());
+}
 
 }

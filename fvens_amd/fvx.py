@@ -301,6 +301,17 @@ class FlowFV:
             check(rc)
         return out
 
+    def surface_data(self, u, wall_tag=2):
+        """(nfaces, 4) array of face centre x, y, Cp and Cf on the wall marker, in boundary-face
+        order (computeSurfaceData's output)."""
+        n = C.c_int()
+        check(lib().fvx_surface_data(self.h, _dptr(u, 4 * self.mesh.nelem), int(wall_tag), None,
+                                     0, C.byref(n), _stream()))
+        out = np.zeros((max(1, n.value), 4))
+        check(lib().fvx_surface_data(self.h, _dptr(u, 4 * self.mesh.nelem), int(wall_tag),
+                                     _ptr(out, C.c_double), n.value, C.byref(n), _stream()))
+        return out[:n.value]
+
     def solve_history(self):
         """Per-step records (step, rel, abs, walltime, lin_walltime, lin_iters, cfl) of the last
         one-instance steady solve on this discretisation."""

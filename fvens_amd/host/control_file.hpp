@@ -5,9 +5,12 @@
  * line, and comments from ';' to the end of the line. parse_flow_controlfile restates the
  * reference's parse_flow_controlfile (src/utilities/controlparser.cpp:60-216, boundary
  * conditions :248-291, boundary names src/spatial/abctypemap.cpp:15-31) for the steady subset:
- * the same keys, defaults and upper-casing of option strings. Differences, both deliberate:
- * spatial_discretization.limiter_parameter is read (the reference never parses it and leaves
- * the parameter uninitialised), and unsteady or RANS settings are rejected.
+ * the same keys, defaults and upper-casing of option strings. Like the reference,
+ * spatial_discretization.limiter_parameter is not read: the reference never parses it, and its
+ * NACA0012 WENO regression (testcases/naca0012/regr-WENO_LeastSquares_HLLC.txt) is reproduced
+ * to 1e-14 with the parameter at zero and not with the file's 20, so zero is the value its runs
+ * used; a caller that wants another value sets limiter_param afterwards. Unsteady and RANS
+ * settings are rejected.
  */
 #ifndef FVENS_GPU_CONTROL_FILE_HPP
 #define FVENS_GPU_CONTROL_FILE_HPP
@@ -247,7 +250,7 @@ inline FlowParserOptions parse_flow_controlfile(const InfoNode& in)
 	o.gradientmethod = upper(in.get("spatial_discretization.gradient_method"));
 	if(o.gradientmethod == "NONE") o.order2 = false;
 	o.limiter = upper(in.get("spatial_discretization.limiter"));
-	o.limiter_param = std::stod(in.get("spatial_discretization.limiter_parameter", "20.0"));
+	o.limiter_param = 0.0;                 // not parsed (see above)
 
 	o.pseudotimetype = upper(in.get("pseudotime.pseudotime_stepping_type"));
 	o.initcfl = std::stod(in.get("pseudotime.main.cfl_min"));

@@ -256,6 +256,16 @@ public:
 		check(fvx_functionals(h_, u, wall_marker, out, stream));
 		return FlowSolutionFunctionals{ 0.0, out[3], out[0], out[1], out[2] };
 	}
+	/// computeSurfaceData's output matrix on one wall marker: rows (x, y, Cp, Cf) per face.
+	std::vector<double> computeSurfaceData(const double* u, int wall_marker,
+	                                       void* stream = nullptr) const
+	{
+		int n = 0;
+		check(fvx_surface_data(h_, u, wall_marker, nullptr, 0, &n, stream));
+		std::vector<double> out(4*(size_t)n);
+		if(n) check(fvx_surface_data(h_, u, wall_marker, out.data(), n, &n, stream));
+		return out;
+	}
 	/// Step records of the last steady solve on this discretisation (TimingData::convhis,
 	/// src/ode/aodesolver.hpp:46-67).
 	std::vector<fvx_step_record> solve_history() const

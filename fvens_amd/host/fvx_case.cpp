@@ -6,17 +6,20 @@
  * run the first-order starter solve when the file has one (a Tolerance_error there is reported
  * and ignored), run the main solve (implicit or explicit), report a main solve that does not
  * converge and go on, then write the point-data VTU file, the residual history
- * (<log_file_prefix>-residual_history.log) when convergence_history_required is true, and print
+ * (<log_file_prefix>-residual_history.log) when convergence_history_required is true, the
+ * surface files of each listed wall and other marker (<surface_output_file_prefix>-surf_w<m>.out,
+ * -surf_o<m>.out; FlowOutput::exportSurfaceData, src/spatial/aoutput.cpp:181-300), and print
  * the functionals on the first output wall marker. --regression_file checks them like
  * tests/flow_solve.cpp:89-123 (CL to 100 x tol, CDp and CDsf to tol, relative).
  *
  *   fvx_case FILE.ctrl [--mesh_file F] [--regression_file F] [--regression_tol 1e-8]
  *                      [--pc sgs] [--sweeps 16] [--precision f64|f32] [--matrix_free]
- *                      [--rcm] [--ksp_max_it 30] [--ksp_rtol 0.1]
+ *                      [--rcm] [--ksp_max_it 30] [--ksp_rtol 0.1] [--limiter_param 0]
  *
  * The preconditioner, its sweeps and storage precision, matrix-free operation and the RCM
  * ordering stand in for the PETSc options file of the reference (-pc_type, -mesh_reorder,
- * -matrix_free_jacobian).
+ * -matrix_free_jacobian). The limiter parameter (WENO central weight, Venkatakrishnan K) is
+ * zero as in the reference's runs unless --limiter_param gives it.
  *
  * Flag mode (no control file) keeps the inviscid airfoil case of
  * testcases/naca0012/transonic-sanity-test-muscl.ctrl:
@@ -80,7 +83,8 @@ int regression(const std::string& file, double tol, const FlowSolutionFunctional
 int run_control_file(const std::string& ctrl, std::map<std::string, std::string>& opt, bool rcm,
                      bool mf)
 {
-	const FlowParserOptions o = parse_flow_controlfile(read_info_file(ctrl));
+	FlowParserOptions o = parse_flow_controlfile(read_info_file(ctrl));
+	if(opt.count("limiter_param")) o.limiter_param = std::stod(opt["limiter_param"]);
 	const std::string meshfile = opt.count("mesh_file") ? opt["mesh_file"] : o.meshfile;
 	MeshData md = read_mesh(meshfile);
 	if(rcm) md.reorder_rcm();
@@ -135,6 +139,15 @@ int run_control_file(const std::string& ctrl, std::map<std::string, std::string>
 	if(!o.vtu_output_file.empty()) {
 		const PointData pd = postprocess_point(md, uf, m.areas(), pconf.gamma, pconf.Minf);
 		write_vtu_point_data(o.vtu_output_file, md, pd);
+	}
+	for(int w : o.lwalls) {
+		const FlowSolutionFunctionals fw = prob.computeFunctionals(u.p, w);
+		write_wall_surface_file(o.surfnameprefix, w, prob.computeSurfaceData(u.p, w), fw);
+		std::printf("FlowOutput: CL = %g   CDp = %g    CDf = %g\n", fw.CL, fw.CDp, fw.CDsf);
+	}
+	if(!o.lothers.empty()) {
+		const std::vector<int> host = boundary_host_cells(md);
+		for(int w : o.lothers) write_other_surface_file(o.surfnameprefix, w, md, host, uf);
 	}
 	const int wall = o.lwalls.empty() ? 2 : o.lwalls[0];
 	const FlowSolutionFunctionals f = prob.computeFunctionals(u.p, wall);

@@ -11,11 +11,16 @@
  *   (writeScalarsVectorToVtu_PointData, src/spatial/aoutput.cpp:427-551).
  * - convergence history: the column layout of writeConvergenceHistoryHeader /
  *   writeStepToConvergenceHistory (src/spatial/aoutput.cpp:618-638).
+ * - surface files of FlowOutput::exportSurfaceData (src/spatial/aoutput.cpp:181-300):
+ *   <prefix>-surf_w<marker>.out with x, y, Cp, Cf per wall face and the marker's CL, CDp, CDf,
+ *   and <prefix>-surf_o<marker>.out with x, y and the host cell's velocity per face of the
+ *   other listed markers. Faces are written in boundary-face order of the mesh.
  */
 #ifndef FVENS_GPU_OUTPUT_HPP
 #define FVENS_GPU_OUTPUT_HPP
 
 #include "fvens_gpu.hpp"
+#include <algorithm>
 #include <cmath>
 #include <fstream>
 #include <iomanip>
@@ -103,6 +108,58 @@ inline void write_vtu_point_data(const std::string& fname, const MeshData& m, co
 	    << "\t\t\t<DataArray type=\"Int32\" Name=\"types\" Format=\"ascii\">\n";
 	for(int e = 0; e < ne; e++) out << "\t\t\t\t" << (m.nnode[e] == 4 ? 9 : 5) << '\n';
 	out << "\t\t\t</DataArray>\n\t\t</Cells>\n\t</Piece>\n</UnstructuredGrid>\n</VTKFile>";
+}
+
+/// Wall-marker surface file; rows: computeSurfaceData's [nfaces][4] (x, y, Cp, Cf).
+inline void write_wall_surface_file(const std::string& prefix, int marker,
+                                    const std::vector<double>& rows,
+                                    const FlowSolutionFunctionals& f)
+{
+	const std::string fname = prefix + "-surf_w" + std::to_string(marker) + ".out";
+	std::ofstream out(fname);
+	if(!out) throw std::runtime_error("cannot write " + fname);
+	out << "#  x \t y \t Cp  \t Cf \n";
+	for(size_t i = 0; i < rows.size()/4; i++) {
+		for(int j = 0; j < 4; j++) out << "  " << rows[4*i + j];
+		out << '\n';
+	}
+	out << "# Cl      Cdp      Cdf\n";
+	out << "# " << f.CL << "  " << f.CDp << "  " << f.CDsf << '\n';
+}
+
+/// Host cell of each boundary face: the cell having both of the face's points.
+inline std::vector<int> boundary_host_cells(const MeshData& m)
+{
+	std::vector<std::vector<int>> cells_of(m.npoin());
+	for(int e = 0; e < m.nelem(); e++)
+		for(int k = 0; k < m.nnode[e]; k++) cells_of[m.inpoel[4*(size_t)e + k]].push_back(e);
+	std::vector<int> host(m.nbface(), -1);
+	for(int b = 0; b < m.nbface(); b++) {
+		const std::vector<int>& a = cells_of[m.bface[2*(size_t)b]];
+		const std::vector<int>& c = cells_of[m.bface[2*(size_t)b + 1]];
+		for(int e : a)
+			if(std::find(c.begin(), c.end(), e) != c.end()) { host[b] = e; break; }
+		if(host[b] < 0) throw std::runtime_error("boundary face without a host cell");
+	}
+	return host;
+}
+
+/// Other-marker surface file: face centre and velocity of the host cell. u: host [nelem][4].
+inline void write_other_surface_file(const std::string& prefix, int marker, const MeshData& m,
+                                     const std::vector<int>& host, const std::vector<double>& u)
+{
+	const std::string fname = prefix + "-surf_o" + std::to_string(marker) + ".out";
+	std::ofstream out(fname);
+	if(!out) throw std::runtime_error("cannot write " + fname);
+	out << "#   x         y          u           v\n";
+	for(int b = 0; b < m.nbface(); b++) {
+		if(m.btag[b] != marker) continue;
+		const int p0 = m.bface[2*(size_t)b], p1 = m.bface[2*(size_t)b + 1];
+		const double* w = &u[4*(size_t)host[b]];
+		out << "  " << 0.5*(m.coords[2*(size_t)p0] + m.coords[2*(size_t)p1])
+		    << "  " << 0.5*(m.coords[2*(size_t)p0 + 1] + m.coords[2*(size_t)p1 + 1])
+		    << "  " << w[1]/w[0] << "  " << w[2]/w[0] << '\n';
+	}
 }
 
 inline void write_convergence_history_header(std::ostream& o)

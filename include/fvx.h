@@ -129,6 +129,14 @@ fvx_status fvx_get_primitive_gradients(fvx_disc d, const double* u, double* grad
 fvx_status fvx_functionals(fvx_disc d, const double* u, int wall_tag, double* out,
                            void* stream);
 
+/* Surface data on one wall marker: per boundary face carrying the marker, in boundary-face
+ * order, the face centre (x, y), Cp and Cf from the cell-centred state -- the output matrix of
+ * FlowFV_base::computeSurfaceData (src/spatial/flow_spatial.cpp:130-310) that
+ * FlowOutput::exportSurfaceData writes (src/spatial/aoutput.cpp:181-240). Host out[cap][4];
+ * *nfaces receives the number of faces (out may be NULL to query it). Synchronises the stream. */
+fvx_status fvx_surface_data(fvx_disc d, const double* u, int wall_tag, double* out, int cap,
+                            int* nfaces, void* stream);
+
 /* ---- implicit solve: system matrix, preconditioners, Krylov solver, pseudo-time loop ---- */
 
 /* Storage precision of matrices and preconditioner factors; vectors are always double. */

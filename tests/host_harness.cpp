@@ -6,6 +6,7 @@
 #include "output.hpp"
 #include "su2_reader.hpp"
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 using namespace fvens_gpu;
@@ -79,6 +80,24 @@ int main(int argc, char** argv)
 				u[4*(size_t)e + 3] = p/0.4 + 0.5*rho*(vx*vx + vy*vy);
 			}
 			write_vtu_point_data(argv[3], m, postprocess_point(m, u, area, 1.4, 0.5));
+			return 0;
+		}
+		if(!std::strcmp(argv[1], "surf") && argc > 4) {
+			// surface file of the "other" marker argv[4] under prefix argv[3], for the state
+			// rho = 1 + 0.001 (e mod 17), velocity (0.5, 0.01 (e mod 5))
+			const MeshData m = read_mesh(argv[2]);
+			const int ne = m.nelem();
+			std::vector<double> u(4*(size_t)ne);
+			for(int e = 0; e < ne; e++) {
+				const double rho = 1.0 + 0.001*(e % 17);
+				u[4*(size_t)e] = rho;
+				u[4*(size_t)e + 1] = rho*0.5;
+				u[4*(size_t)e + 2] = rho*0.01*(e % 5);
+				u[4*(size_t)e + 3] = 2.5;
+			}
+			const std::vector<int> host = boundary_host_cells(m);
+			write_other_surface_file(argv[3], std::atoi(argv[4]), m, host, u);
+			for(int h : host) std::printf("%d\n", h);
 			return 0;
 		}
 	}

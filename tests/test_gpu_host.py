@@ -4,9 +4,11 @@ and checks CL / CDp against testcases/naca0012/regr-MUSCL_LeastSquares_HLLC.txt
 (tests/flow_solve.cpp:89-123). The regression values are converged to 1e-7 only and differ from
 the converged solution by ~5e-8 in CDp, so CDp is held to 1e-7 (CL to 100x that, as the
 reference's test scales it)."""
+import math
 import os
 import subprocess
 
+import numpy as np
 import pytest
 
 from conftest import ROOT, golden
@@ -66,3 +68,29 @@ def test_cpp_case_driver_runs_the_reference_control_file(torch_cuda, tmp_path):
     # residual met the file's tolerance, 1e-6
     assert float(rows[0][1]) == 0.0 and float(rows[-1][1]) <= -5.999
     assert all(float(r[1]) > -6.0 for r in rows[:-1])
+    # surface file of wall marker 2 (exportSurfaceData): one row (x, y, Cp, Cf) per wall face,
+    # then the marker's CL, CDp, CDf, which are the functionals the driver printed
+    surf = (tmp_path / "vnaca-surf_w2.out").read_text().splitlines()
+    assert surf[0].startswith("#  x")
+    data = np.array([ln.split() for ln in surf if not ln.startswith("#")], float)
+    assert data.shape[1] == 4 and len(data) == 200
+    cl, cdp, cdf = (float(v) for v in surf[-1].lstrip("#").split())
+    line = [ln for ln in p.stdout.splitlines() if ln.startswith("CL ")][0].split()
+    assert math.isclose(cl, float(line[1]), rel_tol=1e-5)
+    assert math.isclose(cdp, float(line[3]), rel_tol=1e-5)
+    assert math.isclose(cdf, float(line[5]), rel_tol=1e-5)
+    # the airfoil surface lies within the unit chord
+    assert data[:, 0].min() >= -1e-6 and data[:, 0].max() <= 1.0 + 1e-6
+
+
+def test_cpp_case_driver_runs_the_reference_weno_control_file(torch_cuda, tmp_path):
+    # testcases/naca0012/transonic-sanity-test-weno.ctrl with the reference's CTest command
+    # line (mesh and regression file given): the limiter parameter is left at the reference's
+    # effective zero, and CL / CDp meet the regression file within 1e-6 (CL 100x that)
+    p = subprocess.run([DRIVER, golden(os.path.join("ctrl", "naca0012-sanity-weno.ctrl")),
+                        "--mesh_file", golden("naca0012luo.msh"),
+                        "--regression_file", golden("naca0012_regr_WENO_LeastSquares_HLLC.txt"),
+                        "--regression_tol", "1e-6"],
+                       capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
+    print(p.stdout, p.stderr)
+    assert p.returncode == 0 and "REGRESSION PASSED" in p.stdout

@@ -89,6 +89,8 @@ def test_bump_every_flux_leastsquares(torch_cuda, flux):
 @pytest.mark.parametrize("grad,recon", [("LEASTSQUARES", "VENKATAKRISHNAN"),
                                         ("LEASTSQUARES", "BARTHJESPERSEN"),
                                         ("LEASTSQUARES", "VANALBADA"),
+                                        ("LEASTSQUARES", "WENO"),
+                                        ("GREENGAUSS", "WENO"),
                                         ("GREENGAUSS", "VENKATAKRISHNAN"),
                                         ("NONE", "NONE")])
 def test_bump_reconstructions(torch_cuda, grad, recon):
@@ -112,6 +114,7 @@ def test_fixture_meshes_all_boundary_kinds(torch_cuda, meshname, bcs):
 @pytest.mark.parametrize("recon,grad,constvisc", [("VANALBADA", "LEASTSQUARES", False),
                                                   ("NONE", "GREENGAUSS", True),
                                                   ("VENKATAKRISHNAN", "LEASTSQUARES", False),
+                                                  ("WENO", "LEASTSQUARES", False),
                                                   ("NONE", "NONE", False)])
 def test_viscous_airfoil(torch_cuda, recon, grad, constvisc):
     raw = meshgen.airfoil_omesh(128, 32, 0, 2e-3, 20.0)
@@ -193,3 +196,56 @@ def test_full_size_freestream_and_determinism(torch_cuda):
     torch.cuda.synchronize()
     assert torch.equal(r1, r2)
     assert torch.isfinite(r1).all()
+
+
+def test_surface_data_matches_oracle_gradients(torch_cuda):
+    """Per wall face (x, y, Cp, Cf) of computeSurfaceData (src/spatial/flow_spatial.cpp:130-310):
+    Cp = 2 (p - p_inf) and Cf = 2 mu ((grad v + grad v^T) n) . t with t = (n_y, -n_x), restated
+    here from the oracle's conserved gradients; the rows also integrate to the oracle's CL, CDp
+    and CDsf (length-weighted, divided by the wall length). The case has constant viscosity in
+    its fluxes; the surface data take mu from Sutherland's law all the same, as the reference
+    does (flow_spatial.cpp:226)."""
+    import torch
+    raw = meshgen.airfoil_omesh(128, 32, 0, 2e-3, 20.0)
+    aoa = 2.0
+    phys = dict(VISC_PHYS, aoa_deg=aoa, bcs=AIRFOIL_VISC, constvisc=True)
+    num = dict(flux="ROE", gradient="LEASTSQUARES", recon="VANALBADA")
+    ocfg, pcfg, ncfg = _cfgs(phys, num)
+    od = po.Disc(po.Mesh(raw), ocfg)
+    gm = fvx.Mesh(raw)
+    gd = fvx.FlowFV(gm, pcfg, ncfg)
+    u = states.perturbed_freestream(raw, Minf=0.5, aoa_deg=aoa)
+    rows = gd.surface_data(torch.tensor(u, device="cuda"), wall_tag=2)
+    wall = np.flatnonzero(raw.btag == 2)
+    assert rows.shape == (len(wall), 4) and len(wall) == 128
+    intfac, fm, _, _ = gm.faces()
+    c = intfac[wall, 0]
+    n, ln = fm[wall, :2], fm[wall, 2]
+    mid = 0.5 * (raw.coords[raw.bface[wall, 0]] + raw.coords[raw.bface[wall, 1]])
+    assert np.abs(rows[:, :2] - mid).max() <= 1e-14
+    g = od.cons_gradients(u)[c]
+    w = u[c]
+    p = 0.4 * (w[:, 3] - 0.5 * (w[:, 1] ** 2 + w[:, 2] ** 2) / w[:, 0])
+    pinf = 1.0 / (1.4 * 0.5 ** 2)
+    np.testing.assert_allclose(rows[:, 2], 2 * (p - pinf), rtol=0, atol=1e-13)
+    # velocity gradients dv_a/dx_b from the conserved ones
+    gv = (g[:, 1:3, :] * w[:, 0, None, None] - w[:, 1:3, None] * g[:, 0, None, :]) \
+        / (w[:, 0] ** 2)[:, None, None]
+    s = gv + gv.transpose(0, 2, 1)
+    fo = np.einsum("fab,fb->fa", s, n)
+    t = np.stack([n[:, 1], -n[:, 0]], 1)
+    tem = 1.4 * 0.5 ** 2 * p / w[:, 0]
+    s_ = 110.5 / 288.15
+    mu = (1 + s_) / (tem + s_) * tem ** 1.5 / 5000.0
+    cf = 2.0 * mu * np.sum(fo * t, 1)
+    assert np.abs(rows[:, 3] - cf).max() <= 1e-12 * np.abs(cf).max()
+    ca, sa = np.cos(np.radians(aoa)), np.sin(np.radians(aoa))
+    L = ln.sum()
+    ref = od.functionals(u, walltag=2)
+    cl = np.sum(rows[:, 2] * (-n[:, 0] * sa + n[:, 1] * ca) * ln) / L
+    cdp = np.sum(rows[:, 2] * (n[:, 0] * ca + n[:, 1] * sa) * ln) / L
+    cdf = np.sum(rows[:, 3] * (t[:, 0] * ca + t[:, 1] * sa) * ln) / L
+    for k, v in (("CL", cl), ("CDp", cdp), ("CDsf", cdf)):
+        assert abs(v - ref[k]) <= 1e-12 * max(abs(ref[k]), 1e-3), (k, v, ref[k])
+    # a marker with no faces gives no rows
+    assert gd.surface_data(torch.tensor(u, device="cuda"), wall_tag=99).shape == (0, 4)

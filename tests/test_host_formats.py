@@ -50,7 +50,8 @@ def test_viscous_example_control_file(harness):
     assert d["bc3"].split() == ["6", "7", "10", "290"]
     assert d["lwalls"].split() == ["5", "7"] and d["lothers"].split() == ["4"]
     assert d["invflux"] == "ROE" and d["invfluxjac"] == "ROE"       # "consistent"
-    assert d["limiter"] == "WENO" and float(d["limparam"]) == 20.0
+    # limiter_parameter 20.0 is in the file; the reference does not parse it (zero)
+    assert d["limiter"] == "WENO" and float(d["limparam"]) == 0.0
     assert d["pseudotime"] == "IMPLICIT"
     assert [float(x) for x in d["main"].split()] == [100.0, 2000.0, 1e-5, 500]
     assert [float(x) for x in d["starter"].split()] == [1, 50.0, 500.0, 0.1, 50]
@@ -128,3 +129,28 @@ def test_vtu_point_data(harness, tmp_path):
     offsets = arrays["offsets"].astype(int)
     assert np.array_equal(offsets, np.cumsum(raw.nnode))
     assert set(arrays["types"].astype(int)) <= {5, 9}
+
+
+def test_other_marker_surface_file(harness, tmp_path):
+    # exportSurfaceData's file for a non-wall marker (src/spatial/aoutput.cpp:243-296): face
+    # centre and the host cell's velocity per boundary face with that marker, in boundary order
+    mesh = golden("NACA0012_lam_hybrid_1.msh")
+    raw = gmsh.read_gmsh2(mesh)
+    prefix = str(tmp_path / "case")
+    out = subprocess.run([harness, "surf", mesh, prefix, "4"], check=True, capture_output=True,
+                         text=True).stdout
+    host = np.array(out.split(), int)
+    # the host cell of each boundary face holds both of its points
+    for b in range(raw.nbface):
+        pts = set(raw.inpoel[host[b], :raw.nnode[host[b]]])
+        assert set(raw.bface[b]) <= pts
+    lines = open(prefix + "-surf_o4.out").read().splitlines()
+    assert lines[0].split() == ["#", "x", "y", "u", "v"]
+    rows = np.array([ln.split() for ln in lines[1:]], float)
+    far = np.flatnonzero(raw.btag == 4)
+    assert len(rows) == len(far) == 50
+    mid = 0.5 * (raw.coords[raw.bface[far, 0]] + raw.coords[raw.bface[far, 1]])
+    np.testing.assert_allclose(rows[:, :2], mid, rtol=1e-5, atol=1e-9)
+    e = host[far]
+    np.testing.assert_allclose(rows[:, 2], 0.5, rtol=1e-12)
+    np.testing.assert_allclose(rows[:, 3], 0.01 * (e % 5), rtol=1e-5, atol=1e-12)

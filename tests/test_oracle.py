@@ -272,3 +272,77 @@ def test_uniform_grid_timestep_closed_form():
     vx, vy = np.cos(np.pi / 6), np.sin(np.pi / 6)
     expect = h * h / (2 * (abs(vx) + c) * h + 2 * (abs(vy) + c) * h)
     assert np.allclose(dt, expect, rtol=1e-13)
+
+
+# ---------------------------------------------------------------- WENO reconstruction pin ---
+
+def _weno_cell_residuals(m, d, cfg, u, cells, lam, gam=4.0, eps=1e-5):
+    """Residual of interior cells restated from WENOReconstruction::compute_face_values
+    (src/spatial/limitedlinearreconstruction.cpp:39-98): per cell and primitive variable the
+    gradient average weighted by lam / (|g_i|^2 + eps)^gam (own) and 1 / (|g_j|^2 + eps)^gam
+    (each non-ghost neighbour), linear extrapolation to the face midpoints, then the pinned
+    numerical flux summed with the residual's sign convention (-F len on the lower-index cell,
+    +F len on the higher one)."""
+    ne = m.nelem
+    g = d.prim_gradients(u)
+    w = u.copy()
+    w[:, 1:3] = u[:, 1:3] / u[:, :1]
+    w[:, 3] = 0.4 * (u[:, 3] - 0.5 * (u[:, 1] ** 2 + u[:, 2] ** 2) / u[:, 0])
+
+    def lgrad(c):
+        mag = np.sum(g[c] ** 2, 1) + eps                   # [4]
+        wt = lam / mag ** gam
+        acc, ws = wt[:, None] * g[c], wt.copy()
+        for j in range(int(m.raw.nnode[c])):
+            e = m.esuel[c, j]
+            if e >= ne:
+                continue
+            wj = 1.0 / (np.sum(g[e] ** 2, 1) + eps) ** gam
+            acc += wj[:, None] * g[e]
+            ws += wj
+        return acc / ws[:, None]
+
+    def cons(q):
+        return np.array([q[0], q[0] * q[1], q[0] * q[2],
+                         q[3] / 0.4 + 0.5 * q[0] * (q[1] ** 2 + q[2] ** 2)])
+
+    out = []
+    for c in cells:
+        r = np.zeros(4)
+        for j in range(int(m.raw.nnode[c])):
+            f, e = m.elemface[c, j], m.esuel[c, j]
+            lo, hi = min(c, e), max(c, e)
+            ql = w[lo] + lgrad(lo) @ (m.gr[f] - m.rc[lo])
+            qr = w[hi] + lgrad(hi) @ (m.gr[f] - m.rc[hi])
+            assert m.intfac[f, 0] == lo and m.intfac[f, 1] == hi
+            F = po.flux(cfg, "HLLC", cons(ql), cons(qr), m.facemetric[f, :2])
+            r += (-1.0 if c == lo else 1.0) * F * m.facemetric[f, 2]
+        out.append(r)
+    return np.array(out)
+
+
+def test_weno_reconstruction_restated_on_interior_cells():
+    raw = meshgen.mesh_k1()
+    m = po.Mesh(raw)
+    lam = 5.0
+    cfg = po.make_config(Minf=0.5, flux="HLLC", gradient="LEASTSQUARES", recon="WENO",
+                         limiter_param=lam,
+                         bcs=[(2, "slipwall"), (3, "inflowoutflow"), (4, "inflowoutflow")])
+    d = po.Disc(m, cfg)
+    u = states.perturbed_freestream(raw, Minf=0.5, L=3.0)
+    r, _ = d.residual(u)
+    s = d.flux_magnitude(u)
+    ne = m.nelem
+    # cells two layers away from any boundary: every face and every neighbour is interior
+    inner = [c for c in range(ne)
+             if all(m.esuel[c, j] < ne and all(m.esuel[m.esuel[c, j], k] < ne
+                                              for k in range(int(raw.nnode[m.esuel[c, j]])))
+                    for j in range(int(raw.nnode[c])))]
+    cells = np.random.default_rng(7).choice(inner, 40, replace=False)
+    want = _weno_cell_residuals(m, d, cfg, u, cells, lam)
+    assert np.all(np.abs(r[cells] - want) <= 1e-12 * s[cells])
+    # a plausible mistake (squared instead of fourth-power weights, or a dropped central
+    # weight) is caught by the same comparison
+    for bad in (_weno_cell_residuals(m, d, cfg, u, cells, lam, gam=2.0),
+                _weno_cell_residuals(m, d, cfg, u, cells, 1.0)):
+        assert np.abs(r[cells] - bad).max() > 1e3 * 1e-12 * s[cells].max()

@@ -231,14 +231,13 @@ def test_ilu0_matches_dense_restricted_factorisation():
 NACA_BCS = [(2, "slipwall"), (4, "farfield")]
 
 
-def naca_regression_solve(pc="sgs", tol=1e-7, maxiter=170):
+def naca_regression_solve(pc="sgs", tol=1e-7, maxiter=170, recon="VANALBADA", lim=20.0):
     raw = gmsh.read_gmsh2(golden("naca0012luo.msh"))
     m = po.Mesh(raw)
     d1 = po.Disc(m, po.make_config(Minf=0.8, aoa_deg=1.25, bcs=NACA_BCS, flux="HLLC",
                                    gradient="NONE", order2=False))
     d2 = po.Disc(m, po.make_config(Minf=0.8, aoa_deg=1.25, bcs=NACA_BCS, flux="HLLC",
-                                   gradient="LEASTSQUARES", recon="VANALBADA",
-                                   limiter_param=20.0))
+                                   gradient="LEASTSQUARES", recon=recon, limiter_param=lim))
     u = np.tile(d1.freestream(), (raw.nelem, 1))
     u, s1 = d1.steady_solve(u, cflinit=50.0, cflfin=1000.0, tol=1e-1, maxiter=20,
                             fullupdate=0, min_relax=0.2, pc=pc)
@@ -253,6 +252,18 @@ def test_naca0012_reference_regression():
     with open(golden("naca0012_regr_MUSCL_LeastSquares_HLLC.txt")) as fh:
         CL, CDp = (float(t) for t in fh.read().split()[:2])
     assert abs(f["CL"] - CL) / abs(CL) <= 1e-6
+    assert abs(f["CDp"] - CDp) / abs(CDp) <= 1e-8
+
+
+def test_naca0012_weno_reference_regression():
+    # testcases/naca0012/transonic-sanity-test-weno.ctrl: 100 main steps at most to 1e-7. The
+    # reference never parses limiter_parameter; its regression values are those of a zero WENO
+    # central weight (lambda = 0 reproduces them to 1e-14, the file's 20 misses CL by 1e-2)
+    _, s1, s2, f = naca_regression_solve(maxiter=100, recon="WENO", lim=0.0)
+    assert s2["converged"] and s2["steps"] <= 100
+    with open(golden("naca0012_regr_WENO_LeastSquares_HLLC.txt")) as fh:
+        CL, CDp = (float(t) for t in fh.read().split()[:2])
+    assert abs(f["CL"] - CL) / abs(CL) <= 1e-8
     assert abs(f["CDp"] - CDp) / abs(CDp) <= 1e-8
 
 
