@@ -35,6 +35,33 @@ fvx_status fvx_mesh_rcm(const fvx_mesh_desc* desc, int* perm)
 	});
 }
 
+fvx_status fvx_mesh_lines(const fvx_mesh_desc* desc, double threshold, int* nlines,
+                          int* line_ptr, int* line_cells)
+{
+	return guarded([&] {
+		require(desc && nlines && line_ptr && line_cells, "null argument");
+		std::vector<std::vector<int>> lines;
+		std::vector<int> celline;
+		fvx::find_lines(*desc, threshold, lines, celline);
+		*nlines = (int)lines.size();
+		line_ptr[0] = 0;
+		int k = 0;
+		for(size_t l = 0; l < lines.size(); l++) {
+			for(int c : lines[l]) line_cells[k++] = c;
+			line_ptr[l+1] = k;
+		}
+	});
+}
+
+fvx_status fvx_mesh_line_order(const fvx_mesh_desc* desc, double threshold, int graph_ordering,
+                               int* perm)
+{
+	return guarded([&] {
+		require(desc && perm, "null argument");
+		fvx::line_order(*desc, threshold, graph_ordering, perm);
+	});
+}
+
 fvx_status fvx_mesh_create(const fvx_mesh_desc* desc, fvx_mesh* out)
 {
 	return guarded([&] {
@@ -198,6 +225,18 @@ fvx_status fvx_pc_create(fvx_mesh m, int type, int precision, fvx_pc* out)
 		require(m && out, "null argument");
 		std::unique_ptr<fvx_pc_s> p(new fvx_pc_s);
 		fvx::pc_init(*p, m, type, precision);
+		*out = p.release();
+	});
+}
+
+fvx_status fvx_pc_create_lines(fvx_mesh m, double threshold, int precision, fvx_pc* out)
+{
+	return guarded([&] {
+		require(m && out, "null argument");
+		require(threshold > 0, "line threshold must be positive");
+		std::unique_ptr<fvx_pc_s> p(new fvx_pc_s);
+		p->line_threshold = threshold;
+		fvx::pc_init(*p, m, FVX_PC_LINE, precision);
 		*out = p.release();
 	});
 }

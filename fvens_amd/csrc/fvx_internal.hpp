@@ -187,6 +187,13 @@ struct fvx_pc_s {
 	std::vector<int> colour, order, coff;
 	fvx::DBuf<int> d_colour, d_order;
 	fvx::DBuf<unsigned char> d_lowmask;
+	// line Gauss-Seidel: vertices (lines, then the other cells) listed by colour in order /
+	// coff, their cells, and per cell the slots of its line neighbours
+	double line_threshold = 10.0;
+	int nvert = 0;
+	std::vector<int> vptr, vcells;
+	fvx::DBuf<int> d_vptr, d_vcells;
+	fvx::DBuf<unsigned char> d_lineinfo;
 	fvx::DBuf<double> v64;
 	fvx::DBuf<float> v32;
 	double* f64() const { return v64.p; }
@@ -196,6 +203,9 @@ struct fvx_pc_s {
 namespace fvx {
 void build_mesh(fvx_mesh_s& m, const fvx_mesh_desc& d);
 void rcm_order(const fvx_mesh_desc& d, int* perm);
+void find_lines(const fvx_mesh_desc& d, double threshold, std::vector<std::vector<int>>& lines,
+                std::vector<int>& celline);
+void line_order(const fvx_mesh_desc& d, double threshold, int graph_ordering, int* perm);
 void build_disc(fvx_disc_s& d);
 GasParams make_gas(const fvx_physics_cfg& p);
 

@@ -311,6 +311,117 @@ void k_ilu_bwd(int cnt, const int* __restrict__ list, const int* __restrict__ il
 	if(valid) z[4*i + q] = dot4(a, tv);
 }
 
+/* Line Gauss-Seidel. The vertices are the lines of strongly coupled cells and every other cell
+ * on its own; vertex v holds cells vcells[vptr[v] .. vptr[v+1]) in line order, and lineinfo[c]
+ * gives the slot of the cell's line predecessor (bits 0-2) and successor (bits 3-5), 7 for none.
+ * The tridiagonal block system of a line is factored by the block Thomas algorithm:
+ *   D'_0 = D_0,  D'_k = D_k - L_k C_{k-1},  C_k = D'_k^-1 U_k,
+ * with L_k the block coupling cell k to k-1 and U_k the block coupling k to k+1. The factor
+ * array keeps the matrix layout: slot 0 holds D'_k^-1, the successor slot C_k, every other
+ * slot the matrix block. */
+__device__ __forceinline__ int line_prev(unsigned char li) { return li & 7; }
+__device__ __forceinline__ int line_next(unsigned char li) { return (li >> 3) & 7; }
+
+template <class VTA, class VTF>
+__global__ __launch_bounds__(BLOCK)
+void k_line_factor(int nv, const int* __restrict__ vptr, const int* __restrict__ vcells,
+                   const unsigned char* __restrict__ lineinfo, const int* __restrict__ ilist,
+                   const VTA* __restrict__ vals, VTF* __restrict__ fac, long sa)
+{
+	const long I = inst_of(ilist);
+	vals += I*sa; fac += I*sa;
+	const int v = blockIdx.x*blockDim.x + threadIdx.x;
+	if(v >= nv) return;
+	double Cp[16] = {};                              // C_{k-1}
+	for(int k = vptr[v]; k < vptr[v+1]; k++) {
+		const int c = vcells[k];
+		const unsigned char li = lineinfo[c];
+		const int ps = line_prev(li), ns = line_next(li);
+		const VTA* a = vals + (size_t)c*80;
+		VTF* f = fac + (size_t)c*80;
+		double D[16], B[16], T[16], Dinv[16];
+		load_block<VTA>(a, D);
+		for(int s = 0; s < 4; s++) {
+			if(s == ns) continue;
+			load_block<VTA>(a + 16*(1 + s), B);
+			put_block<VTF>(f + 16*(1 + s), B);
+			if(s == ps) {
+				gemm4(B, Cp, T);
+				for(int i = 0; i < 16; i++) D[i] -= T[i];
+			}
+		}
+		invert4(D, Dinv);
+		put_block<VTF>(f, Dinv);
+		if(ns < 4) {
+			load_block<VTA>(a + 16*(1 + ns), B);
+			gemm4(Dinv, B, Cp);
+			put_block<VTF>(f + 16*(1 + ns), Cp);
+		}
+	}
+}
+
+/// One colour of a line Gauss-Seidel pass, one thread per vertex: right-hand side
+/// r_c - sum over slots outside the line of A z (current z with ALL, lower colours only in the
+/// first forward pass from z = 0), then the forward and backward Thomas sweeps along the line.
+/// z of the vertex's cells holds the forward result between the two sweeps.
+template <class VTF, bool ALL>
+__global__ __launch_bounds__(BLOCK)
+void k_line_colour(int cnt, const int* __restrict__ list, const int* __restrict__ vptr,
+                   const int* __restrict__ vcells, const unsigned char* __restrict__ lineinfo,
+                   const int* __restrict__ ilist, const VTF* __restrict__ fac, long sa,
+                   const int* __restrict__ nbr, const unsigned char* __restrict__ lowmask,
+                   const double* __restrict__ r, long sr, double* __restrict__ z, long sz)
+{
+	const long I = inst_of(ilist);
+	fac += I*sa; r += I*sr; z += I*sz;
+	const int idx = blockIdx.x*blockDim.x + threadIdx.x;
+	if(idx >= cnt) return;
+	const int v = list[idx];
+	const int k0 = vptr[v], k1 = vptr[v+1];
+	double g[4] = {0, 0, 0, 0};
+	for(int k = k0; k < k1; k++) {
+		const int c = vcells[k];
+		const unsigned char li = lineinfo[c];
+		const int ps = line_prev(li), ns = line_next(li);
+		const int4 nb = load_nbr(nbr, c);
+		const int nbs[4] = { nb.x, nb.y, nb.z, nb.w };
+		const int lm = lowmask[c];
+		const VTF* f = fac + (size_t)c*80;
+		double t[4], a[4], zv[4];
+		load4(r + 4*c, t);
+		for(int s = 0; s < 4; s++) {
+			if(nbs[s] < 0 || s == ns) continue;
+			const bool prev = s == ps;
+			if(!prev && !(ALL || ((lm >> s) & 1))) continue;
+			if(prev) for(int i = 0; i < 4; i++) zv[i] = g[i];
+			else load4(z + 4*nbs[s], zv);
+			for(int i = 0; i < 4; i++) {
+				load_row<VTF>(f + 16*(1 + s) + 4*i, a);
+				t[i] -= dot4(a, zv);
+			}
+		}
+		for(int i = 0; i < 4; i++) {
+			load_row<VTF>(f + 4*i, a);
+			g[i] = dot4(a, t);
+		}
+		store4(z + 4*c, g);
+	}
+	// backward: x_k = g_k - C_k x_{k+1}; g holds x of the last cell
+	for(int k = k1 - 2; k >= k0; k--) {
+		const int c = vcells[k];
+		const int ns = line_next(lineinfo[c]);
+		const VTF* f = fac + (size_t)c*80 + 16*(1 + ns);
+		double gk[4], a[4], x[4];
+		load4(z + 4*c, gk);
+		for(int i = 0; i < 4; i++) {
+			load_row<VTF>(f + 4*i, a);
+			x[i] = gk[i] - dot4(a, g);
+		}
+		store4(z + 4*c, x);
+		for(int i = 0; i < 4; i++) g[i] = x[i];
+	}
+}
+
 /// Block ILU(0) factorisation of the rows of one colour (IKJ form on the matrix pattern,
 /// rows taken in (colour, index) order; one thread per row). For each lower neighbour k in
 /// ascending order: L_ik = A_ik U_kk^-1, then A_ij -= L_ik U_kj for every j > k in the
@@ -648,6 +759,11 @@ static void pc_setup_t(fvx_pc_s& p, const VTA* vals, VTF* fac, const Inst& L, hi
 		hipLaunchKernelGGL((k_invert_diag<VTA, VTF>), dim3(thread_blocks(n), L.nact), dim3(BLOCK),
 		                   0, s, n, L.list, vals, fac, sa, p.type == FVX_PC_SGS ? 1 : 0);
 		FVX_HIP(hipGetLastError());
+	} else if(p.type == FVX_PC_LINE) {
+		hipLaunchKernelGGL((k_line_factor<VTA, VTF>), dim3(thread_blocks(p.nvert), L.nact),
+		                   dim3(BLOCK), 0, s, p.nvert, p.d_vptr.p, p.d_vcells.p, p.d_lineinfo.p,
+		                   L.list, vals, fac, sa);
+		FVX_HIP(hipGetLastError());
 	} else if(p.type == FVX_PC_ILU0) {
 		for(int c = 0; c < p.ncol; c++) {
 			const int cnt = p.coff[c+1] - p.coff[c];
@@ -705,6 +821,32 @@ static void sgs_passes(const fvx_pc_s& p, const VTF* fac, const double* r, long 
 	}
 }
 
+/// One forward and one backward pass of line Gauss-Seidel over the colours of the vertices.
+template <class VTF>
+static void line_passes(const fvx_pc_s& p, const VTF* fac, const double* r, long sr, double* z,
+                        long sz, bool first, const Inst& L, hipStream_t s)
+{
+	const int* nbr = p.m->d_nbr.p;
+	const long sa = mstride(p.m);
+	auto colour = [&](int c, bool all) {
+		const int cnt = p.coff[c+1] - p.coff[c];
+		if(cnt == 0) return;
+		const int* list = p.d_order.p + p.coff[c];
+		const dim3 g(thread_blocks(cnt), L.nact);
+		if(all)
+			hipLaunchKernelGGL((k_line_colour<VTF, true>), g, dim3(BLOCK), 0, s, cnt, list,
+			                   p.d_vptr.p, p.d_vcells.p, p.d_lineinfo.p, L.list, fac, sa, nbr,
+			                   p.d_lowmask.p, r, sr, z, sz);
+		else
+			hipLaunchKernelGGL((k_line_colour<VTF, false>), g, dim3(BLOCK), 0, s, cnt, list,
+			                   p.d_vptr.p, p.d_vcells.p, p.d_lineinfo.p, L.list, fac, sa, nbr,
+			                   p.d_lowmask.p, r, sr, z, sz);
+		FVX_HIP(hipGetLastError());
+	};
+	for(int c = 0; c < p.ncol; c++) colour(c, !first);
+	for(int c = p.ncol - 1; c >= 0; c--) colour(c, true);
+}
+
 /// z = M^-1 r for Jacobi or ILU(0) (one application).
 template <class VTF>
 static void pc_once(const fvx_pc_s& p, const VTF* fac, const double* r, long sr, double* z,
@@ -744,6 +886,10 @@ static void pc_apply_t(const fvx_pc_s& p, const VTF* fac, const double* r, long 
 {
 	if(p.type == FVX_PC_SGS) {
 		for(int k = 0; k < p.sweeps; k++) sgs_passes(p, fac, r, sr, z, sz, k == 0, L, s);
+		return;
+	}
+	if(p.type == FVX_PC_LINE) {
+		for(int k = 0; k < p.sweeps; k++) line_passes(p, fac, r, sr, z, sz, k == 0, L, s);
 		return;
 	}
 	pc_once(p, fac, r, sr, z, sz, L, s);

@@ -1,5 +1,5 @@
-/* mesh.cpp -- host-side mesh preprocessing for libfvx: topology, geometry, RCM ordering and
- * the per-cell stencil coefficients consumed by the device kernels.
+/* mesh.cpp -- host-side mesh preprocessing for libfvx: topology, geometry, RCM and line
+ * orderings, and the per-cell stencil coefficients consumed by the device kernels.
  *
  * Topology and numbering follow UMesh (src/mesh/mesh.cpp): faces are numbered physical
  * boundary first (input order, orientation corrected to point out of the domain, :54-82 and
@@ -214,78 +214,287 @@ void build_mesh(fvx_mesh_s& m, const fvx_mesh_desc& d)
 	m.d_btag.upload(m.btag);
 }
 
-/* Reverse Cuthill-McKee on the cell adjacency graph, started from a pseudo-peripheral cell
- * (George-Liu), neighbours visited by increasing degree. */
+/* Lines of strongly coupled cells (findLines, src/mesh/meshordering.cpp:199-262). Each cell's
+ * real neighbours j carry the weight 1/|x_j - x_i| (vertex-average centres) divided by the
+ * smallest of them, in decreasing order (ties keep local-face order). From the host cell of
+ * every boundary face, in boundary-face order, a line grows while the current cell's largest
+ * weight exceeds the threshold, stepping to the first free neighbour (in weight order) whose
+ * weight exceeds it; lines of one cell are dropped. celline[c] is the line of cell c or -1. */
+void find_lines(const fvx_mesh_desc& d, double threshold, std::vector<std::vector<int>>& lines,
+                std::vector<int>& celline)
+{
+	check_desc(d);
+	const int n = d.nelem;
+	const std::vector<EdgeRec> edges = cell_edges(d);
+	const std::vector<int> nb = neighbours(d, edges);
+	std::vector<double> cx(n), cy(n);
+	for(int c = 0; c < n; c++) {
+		double sx = 0, sy = 0;
+		for(int j = 0; j < d.nnode[c]; j++) {
+			sx += d.coords[2*d.inpoel[c*4+j]];
+			sy += d.coords[2*d.inpoel[c*4+j]+1];
+		}
+		cx[c] = sx/d.nnode[c]; cy[c] = sy/d.nnode[c];
+	}
+	// per cell: (weight, local face) of real neighbours, decreasing weight
+	std::vector<double> wt((size_t)n*4, 0.0);
+	std::vector<int> wface((size_t)n*4, -1), nreal(n, 0);
+	for(int c = 0; c < n; c++) {
+		std::pair<double, int> e[4];
+		int k = 0;
+		double minw = 1e20;
+		for(int j = 0; j < d.nnode[c]; j++) {
+			const int o = nb[(size_t)c*4+j];
+			if(o < 0) continue;
+			const double dx = cx[c] - cx[o], dy = cy[c] - cy[o];
+			e[k] = {1.0/std::sqrt(dx*dx + dy*dy), j};
+			minw = std::min(minw, e[k].first);
+			k++;
+		}
+		for(int j = 0; j < k; j++) e[j].first /= minw;
+		std::stable_sort(e, e + k, [](const std::pair<double, int>& a,
+		                              const std::pair<double, int>& b) { return a.first > b.first; });
+		nreal[c] = k;
+		for(int j = 0; j < k; j++) { wt[(size_t)c*4+j] = e[j].first; wface[(size_t)c*4+j] = e[j].second; }
+	}
+	lines.clear();
+	celline.assign(n, -1);
+	for(int b = 0; b < d.nbface; b++) {
+		const long long key = ekey(d.bface[b*2], d.bface[b*2+1]);
+		auto it = std::lower_bound(edges.begin(), edges.end(), EdgeRec{key, -1, -1});
+		if(it == edges.end() || it->key != key)
+			throw Error(FVX_ERR_INVALID, "boundary face is not an edge of any cell");
+		int cur = it->elem;
+		if(celline[cur] >= 0) continue;
+		std::vector<int> line;
+		for(;;) {
+			if(!(nreal[cur] > 0 && wt[(size_t)cur*4] > threshold)) break;
+			line.push_back(cur);
+			celline[cur] = (int)lines.size();
+			int next = -1;
+			for(int j = 0; j < nreal[cur]; j++) {
+				const int o = nb[(size_t)cur*4 + wface[(size_t)cur*4+j]];
+				if(celline[o] == -1 && wt[(size_t)cur*4+j] > threshold) { next = o; break; }
+			}
+			if(next < 0) break;
+			cur = next;
+		}
+		if(line.size() > 1) lines.push_back(line);
+		else if(line.size() == 1) celline[line[0]] = -1;
+	}
+}
+
+namespace {
+
+/* Reverse Cuthill-McKee of SPARSPAK (GENRCM with FNROOT, ROOTLS, DEGREE and RCM; George and
+ * Liu, Computer Solution of Large Sparse Positive Definite Systems, 1981, ch. 4), which is the
+ * "rcm" ordering PETSc's MatGetOrdering applies to a matrix graph. adj lists each vertex's
+ * neighbours in increasing order (the vertex itself may appear; it changes no decision).
+ * Returns perm[new] = old. */
+struct Graph {
+	std::vector<int> ptr, adj;
+};
+
+void rootls(const Graph& g, int root, std::vector<char>& mask, int& nlvl, std::vector<int>& xls,
+            int* ls)
+{
+	mask[root] = 0;
+	ls[0] = root;
+	nlvl = 0;
+	int lvlend = 0, ccsize = 1;
+	xls.clear();
+	do {
+		const int lbegin = lvlend;
+		lvlend = ccsize;
+		xls.push_back(lbegin);
+		nlvl++;
+		for(int i = lbegin; i < lvlend; i++) {
+			const int node = ls[i];
+			for(int k = g.ptr[node]; k < g.ptr[node+1]; k++) {
+				const int nbr = g.adj[k];
+				if(!mask[nbr]) continue;
+				ls[ccsize++] = nbr;
+				mask[nbr] = 0;
+			}
+		}
+	} while(ccsize > lvlend);
+	xls.push_back(lvlend);
+	for(int i = 0; i < ccsize; i++) mask[ls[i]] = 1;
+}
+
+void fnroot(const Graph& g, int& root, std::vector<char>& mask, std::vector<int>& xls, int* ls)
+{
+	int nlvl;
+	rootls(g, root, mask, nlvl, xls, ls);
+	const int ccsize = xls[nlvl];
+	if(nlvl == 1 || nlvl == ccsize) return;
+	for(;;) {
+		const int jstrt = xls[nlvl-1];
+		int mindeg = ccsize;
+		root = ls[jstrt];
+		if(ccsize - 1 != jstrt) {
+			for(int j = jstrt; j < ccsize; j++) {
+				const int node = ls[j];
+				int ndeg = 0;
+				for(int k = g.ptr[node]; k < g.ptr[node+1]; k++) if(mask[g.adj[k]]) ndeg++;
+				if(ndeg >= mindeg) continue;
+				root = node;
+				mindeg = ndeg;
+			}
+		}
+		int nunlvl;
+		rootls(g, root, mask, nunlvl, xls, ls);
+		if(nunlvl <= nlvl) return;
+		nlvl = nunlvl;
+		if(nlvl >= ccsize) return;
+	}
+}
+
+void degree(const Graph& g, int root, const std::vector<char>& mask, std::vector<int>& deg,
+            std::vector<char>& queued, int& ccsize, int* ls)
+{
+	ls[0] = root;
+	queued[root] = 1;
+	ccsize = 1;
+	int lvlend = 0;
+	do {
+		const int lbegin = lvlend;
+		lvlend = ccsize;
+		for(int i = lbegin; i < lvlend; i++) {
+			const int node = ls[i];
+			int ideg = 0;
+			for(int k = g.ptr[node]; k < g.ptr[node+1]; k++) {
+				const int nbr = g.adj[k];
+				if(!mask[nbr]) continue;
+				ideg++;
+				if(queued[nbr]) continue;
+				queued[nbr] = 1;
+				ls[ccsize++] = nbr;
+			}
+			deg[node] = ideg;
+		}
+	} while(ccsize > lvlend);
+	for(int i = 0; i < ccsize; i++) queued[ls[i]] = 0;
+}
+
+void rcm_component(const Graph& g, int root, std::vector<char>& mask, int* perm, int& ccsize,
+                   std::vector<int>& deg, std::vector<char>& queued)
+{
+	degree(g, root, mask, deg, queued, ccsize, perm);
+	mask[root] = 0;
+	if(ccsize <= 1) return;
+	int lvlend = 0, lnbr = 1;
+	while(lnbr > lvlend) {
+		const int lbegin = lvlend;
+		lvlend = lnbr;
+		for(int i = lbegin; i < lvlend; i++) {
+			const int node = perm[i];
+			const int fnbr = lnbr;
+			for(int k = g.ptr[node]; k < g.ptr[node+1]; k++) {
+				const int nbr = g.adj[k];
+				if(!mask[nbr]) continue;
+				mask[nbr] = 0;
+				perm[lnbr++] = nbr;
+			}
+			// the new neighbours by increasing degree (linear insertion, stable)
+			for(int k = fnbr + 1; k < lnbr; k++) {
+				const int v = perm[k];
+				int l = k - 1;
+				while(l >= fnbr && deg[perm[l]] > deg[v]) { perm[l+1] = perm[l]; l--; }
+				perm[l+1] = v;
+			}
+		}
+	}
+	std::reverse(perm, perm + ccsize);
+}
+
+std::vector<int> genrcm(const Graph& g)
+{
+	const int n = (int)g.ptr.size() - 1;
+	std::vector<char> mask(n, 1), queued(n, 0);
+	std::vector<int> perm(n), xls, deg(n, 0);
+	int num = 0;
+	for(int i = 0; i < n && num < n; i++) {
+		if(!mask[i]) continue;
+		int root = i, ccsize = 0;
+		fnroot(g, root, mask, xls, perm.data() + num);
+		rcm_component(g, root, mask, perm.data() + num, ccsize, deg, queued);
+		num += ccsize;
+	}
+	return perm;
+}
+
+}
+
+/* Reverse Cuthill-McKee of the cell graph (a vertex per cell, edges between face neighbours,
+ * the pattern of the one-variable Jacobian that reorderMeshPetsc orders,
+ * src/mesh/ameshutils.cpp:245-288), by the same GENRCM as PETSc's "rcm". */
 void rcm_order(const fvx_mesh_desc& d, int* perm)
 {
 	check_desc(d);
 	const int n = d.nelem;
 	const std::vector<int> nb = neighbours(d, cell_edges(d));
-	std::vector<int> deg(n, 0);
-	for(int c = 0; c < n; c++)
-		for(int j = 0; j < 4; j++) if(nb[(size_t)c*4+j] >= 0) deg[c]++;
-	std::vector<int> level(n, -1), order;
-	order.reserve(n);
-	std::vector<char> done(n, 0);
-	auto bfs_levels = [&](int s, int& last_min) {
-		std::fill(level.begin(), level.end(), -1);
-		std::queue<int> q;
-		q.push(s); level[s] = 0;
-		int maxl = 0;
-		last_min = s;
-		while(!q.empty()) {
-			const int c = q.front(); q.pop();
-			if(level[c] > maxl || (level[c] == maxl && deg[c] < deg[last_min])) {
-				maxl = level[c]; last_min = c;
-			}
-			for(int j = 0; j < 4; j++) {
-				const int o = nb[(size_t)c*4+j];
-				if(o >= 0 && level[o] < 0) { level[o] = level[c] + 1; q.push(o); }
-			}
-		}
-		return maxl;
-	};
-	for(int seed = 0; seed < n; seed++) {
-		if(done[seed]) continue;
-		// lowest-degree cell of this component as the first guess
-		int s = seed;
-		{
-			std::queue<int> q;
-			q.push(seed);
-			level.assign(n, -1); level[seed] = 0;
-			while(!q.empty()) {
-				const int c = q.front(); q.pop();
-				if(deg[c] < deg[s]) s = c;
-				for(int j = 0; j < 4; j++) {
-					const int o = nb[(size_t)c*4+j];
-					if(o >= 0 && level[o] < 0) { level[o] = 0; q.push(o); }
-				}
-			}
-		}
-		int far = s;
-		int ecc = bfs_levels(s, far);
-		for(int it = 0; it < 8; it++) {
-			int far2 = far;
-			const int e2 = bfs_levels(far, far2);
-			if(e2 <= ecc) break;
-			s = far; ecc = e2; far = far2;
-		}
-		std::queue<int> q;
-		q.push(s); done[s] = 1;
-		while(!q.empty()) {
-			const int c = q.front(); q.pop();
-			order.push_back(c);
-			int cand[4], nc = 0;
-			for(int j = 0; j < 4; j++) {
-				const int o = nb[(size_t)c*4+j];
-				if(o >= 0 && !done[o]) { cand[nc++] = o; done[o] = 1; }
-			}
-			std::sort(cand, cand + nc, [&](int a, int b) {
-				return deg[a] != deg[b] ? deg[a] < deg[b] : a < b; });
-			for(int k = 0; k < nc; k++) q.push(cand[k]);
-		}
+	Graph g;
+	g.ptr.assign(1, 0);
+	for(int c = 0; c < n; c++) {
+		int row[5], k = 0;
+		row[k++] = c;
+		for(int j = 0; j < 4; j++) if(nb[(size_t)c*4+j] >= 0) row[k++] = nb[(size_t)c*4+j];
+		std::sort(row, row + k);
+		g.adj.insert(g.adj.end(), row, row + k);
+		g.ptr.push_back((int)g.adj.size());
 	}
-	for(int i = 0; i < n; i++) perm[i] = order[n-1-i];
+	const std::vector<int> p = genrcm(g);
+	std::copy(p.begin(), p.end(), perm);
+}
+
+/* Cell order of lineReorder / getHybridLineOrdering (src/mesh/meshordering.cpp:33-128,
+ * 264-412). graph_ordering 0: the lines in order, then the other cells in index order.
+ * 1: the same through the line-point graph in natural order. 2: the line-point graph (a vertex
+ * per line, then one per remaining cell; edges between vertices holding adjacent cells)
+ * ordered by reverse Cuthill-McKee, each line vertex expanded to its cells. */
+void line_order(const fvx_mesh_desc& d, double threshold, int graph_ordering, int* perm)
+{
+	std::vector<std::vector<int>> lines;
+	std::vector<int> celline;
+	find_lines(d, threshold, lines, celline);
+	const int n = d.nelem, nl = (int)lines.size();
+	std::vector<int> points, pidx(n, -1);
+	for(int c = 0; c < n; c++) if(celline[c] < 0) { pidx[c] = nl + (int)points.size(); points.push_back(c); }
+	const int nv = nl + (int)points.size();
+	std::vector<int> vorder(nv);
+	std::iota(vorder.begin(), vorder.end(), 0);
+	if(graph_ordering == 2) {
+		const std::vector<int> nb = neighbours(d, cell_edges(d));
+		auto vert = [&](int c) { return celline[c] >= 0 ? celline[c] : pidx[c]; };
+		std::vector<std::vector<int>> rows(nv);
+		for(int v = 0; v < nv; v++) {
+			const bool isline = v < nl;
+			const std::vector<int> cells = isline ? lines[v] : std::vector<int>{points[v - nl]};
+			rows[v].push_back(v);
+			for(int c : cells)
+				for(int j = 0; j < d.nnode[c]; j++) {
+					const int o = nb[(size_t)c*4+j];
+					if(o >= 0) rows[v].push_back(vert(o));
+				}
+			std::sort(rows[v].begin(), rows[v].end());
+			rows[v].erase(std::unique(rows[v].begin(), rows[v].end()), rows[v].end());
+		}
+		Graph g;
+		g.ptr.assign(1, 0);
+		for(int v = 0; v < nv; v++) {
+			g.adj.insert(g.adj.end(), rows[v].begin(), rows[v].end());
+			g.ptr.push_back((int)g.adj.size());
+		}
+		vorder = genrcm(g);
+	}
+	else if(graph_ordering != 0 && graph_ordering != 1)
+		throw Error(FVX_ERR_INVALID, "unknown line-graph ordering");
+	int k = 0;
+	for(int v : vorder) {
+		if(v < nl) for(int c : lines[v]) perm[k++] = c;
+		else perm[k++] = points[v - nl];
+	}
 }
 
 /* Stencil coefficients: both gradient schemes become grad_i = sum_j c_ij (w_j - w_i) over the

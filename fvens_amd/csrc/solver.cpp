@@ -79,9 +79,79 @@ void matrix_export_bsr(const fvx_matrix_s& A, int* rowptr, int* colind, double* 
 
 /* ------------------------------------------------------------ preconditioner ----------- */
 
+/* Vertices of line Gauss-Seidel: the lines of findLines at p.line_threshold, then every cell
+ * outside them on its own (the line-point graph of getHybridLineOrdering,
+ * src/mesh/meshordering.cpp:264-385). Vertices are coloured greedily in vertex order so that
+ * no two vertices of one colour hold adjacent cells; p.colour gets each cell's vertex colour,
+ * p.order the vertices by colour. */
+static void line_vertices(fvx_pc_s& p, const fvx_mesh_s& m)
+{
+	const int n = m.nelem;
+	const fvx_mesh_desc d{ m.npoin, m.nelem, m.nbface, m.coords.data(), m.inpoel.data(),
+	                       m.nnode.data(), m.bface.data(), m.btag.data() };
+	std::vector<std::vector<int>> lines;
+	std::vector<int> celline;
+	find_lines(d, p.line_threshold, lines, celline);
+	p.vptr.assign(1, 0);
+	p.vcells.clear();
+	std::vector<int> vert(n, -1);
+	for(const auto& l : lines) {
+		for(int c : l) { vert[c] = (int)p.vptr.size() - 1; p.vcells.push_back(c); }
+		p.vptr.push_back((int)p.vcells.size());
+	}
+	for(int c = 0; c < n; c++)
+		if(celline[c] < 0) { vert[c] = (int)p.vptr.size() - 1; p.vcells.push_back(c); p.vptr.push_back((int)p.vcells.size()); }
+	p.nvert = (int)p.vptr.size() - 1;
+	// slots of each cell's predecessor and successor in its line (7: none)
+	std::vector<unsigned char> info(n, 7 | (7 << 3));
+	for(int v = 0; v < p.nvert; v++)
+		for(int k = p.vptr[v]; k < p.vptr[v+1]; k++) {
+			const int c = p.vcells[k];
+			int ps = 7, ns = 7;
+			for(int s = 0; s < 4; s++) {
+				if(!interior_slot(m, c, s)) continue;
+				const int o = m.esuel[(size_t)c*4 + s];
+				if(k > p.vptr[v] && o == p.vcells[k-1]) ps = s;
+				if(k + 1 < p.vptr[v+1] && o == p.vcells[k+1]) ns = s;
+			}
+			if((k > p.vptr[v] && ps == 7) || (k + 1 < p.vptr[v+1] && ns == 7))
+				throw Error(FVX_ERR_INVALID, "consecutive cells of a line are not neighbours");
+			info[c] = (unsigned char)(ps | (ns << 3));
+		}
+	// greedy colouring of the vertices in vertex order
+	std::vector<int> vcol(p.nvert, 0);
+	p.ncol = 1;
+	for(int v = 0; v < p.nvert; v++) {
+		unsigned long long used = 0;
+		for(int k = p.vptr[v]; k < p.vptr[v+1]; k++) {
+			const int c = p.vcells[k];
+			for(int s = 0; s < 4; s++) {
+				if(!interior_slot(m, c, s)) continue;
+				const int w = vert[m.esuel[(size_t)c*4 + s]];
+				if(w < v) used |= 1ull << vcol[w];
+			}
+		}
+		int col = 0;
+		while(col < 63 && (used & (1ull << col))) col++;
+		if(col == 63) throw Error(FVX_ERR_INVALID, "line-point graph needs more than 63 colours");
+		vcol[v] = col;
+		p.ncol = std::max(p.ncol, col + 1);
+	}
+	for(int c = 0; c < n; c++) p.colour[c] = vcol[vert[c]];
+	p.coff.assign(p.ncol + 1, 0);
+	for(int v = 0; v < p.nvert; v++) p.coff[vcol[v] + 1]++;
+	for(int k = 0; k < p.ncol; k++) p.coff[k+1] += p.coff[k];
+	p.order.assign(p.nvert, 0);
+	std::vector<int> fill(p.coff.begin(), p.coff.end() - 1);
+	for(int v = 0; v < p.nvert; v++) p.order[fill[vcol[v]]++] = v;
+	p.d_vptr.upload(p.vptr);
+	p.d_vcells.upload(p.vcells);
+	p.d_lineinfo.upload(info);
+}
+
 void pc_init(fvx_pc_s& p, const fvx_mesh_s* m, int type, int prec, int ninst)
 {
-	if(type < FVX_PC_NONE || type > FVX_PC_ILU0)
+	if(type < FVX_PC_NONE || type > FVX_PC_LINE)
 		throw Error(FVX_ERR_INVALID, "unknown preconditioner type");
 	if(prec != FVX_PREC_F64 && prec != FVX_PREC_F32)
 		throw Error(FVX_ERR_INVALID, "unknown preconditioner precision");
@@ -108,12 +178,16 @@ void pc_init(fvx_pc_s& p, const fvx_mesh_s* m, int type, int prec, int ninst)
 			p.ncol = std::max(p.ncol, col + 1);
 		}
 	}
-	p.coff.assign(p.ncol + 1, 0);
-	for(int c = 0; c < n; c++) p.coff[p.colour[c] + 1]++;
-	for(int k = 0; k < p.ncol; k++) p.coff[k+1] += p.coff[k];
-	p.order.assign(n, 0);
-	std::vector<int> fill(p.coff.begin(), p.coff.end() - 1);
-	for(int c = 0; c < n; c++) p.order[fill[p.colour[c]]++] = c;
+	if(type == FVX_PC_LINE) {
+		line_vertices(p, *m);
+	} else {
+		p.coff.assign(p.ncol + 1, 0);
+		for(int c = 0; c < n; c++) p.coff[p.colour[c] + 1]++;
+		for(int k = 0; k < p.ncol; k++) p.coff[k+1] += p.coff[k];
+		p.order.assign(n, 0);
+		std::vector<int> fill(p.coff.begin(), p.coff.end() - 1);
+		for(int c = 0; c < n; c++) p.order[fill[p.colour[c]]++] = c;
+	}
 	std::vector<unsigned char> low(n, 0);
 	for(int c = 0; c < n; c++)
 		for(int k = 0; k < 4; k++)
@@ -144,7 +218,7 @@ void pc_set_sweeps(fvx_pc_s& p, int sweeps)
 {
 	if(sweeps < 1 || sweeps > 64) throw Error(FVX_ERR_INVALID, "sweep count must be in [1, 64]");
 	p.sweeps = sweeps;
-	if(sweeps > 1 && p.type != FVX_PC_SGS && p.type != FVX_PC_NONE) {
+	if(sweeps > 1 && p.type != FVX_PC_SGS && p.type != FVX_PC_LINE && p.type != FVX_PC_NONE) {
 		const size_t N = 4*(size_t)p.m->nelem*p.ninst;
 		if(p.t1.n != N) { p.t1.alloc(N); p.t2.alloc(N); }
 	}
@@ -441,7 +515,10 @@ void steady_core(fvx_disc_s& d, const fvx_solver_cfg& c, double* u, int ninst,
 			matrix_init(w.A, &m, c.jac_precision, ninst);
 			w.jprec = c.jac_precision;
 		}
-		if(w.pctype != c.pc || w.pcprec != c.pc_precision) {
+		const double thr = c.line_threshold > 0 ? c.line_threshold : 10.0;
+		if(w.pctype != c.pc || w.pcprec != c.pc_precision
+		   || (c.pc == FVX_PC_LINE && w.P.line_threshold != thr)) {
+			w.P.line_threshold = thr;
 			pc_init(w.P, &m, c.pc, c.pc_precision, ninst);
 			w.pctype = c.pc;
 			w.pcprec = c.pc_precision;

@@ -154,6 +154,29 @@ def rcm_permutation(raw: RawMesh) -> np.ndarray:
     return perm
 
 
+def find_lines(raw: RawMesh, threshold: float):
+    """Lines of strongly coupled cells (findLines): a list of cell-index arrays."""
+    d, keep = _desc(raw)
+    n = C.c_int()
+    ptr = np.zeros(raw.nelem // 2 + 2, np.int32)
+    cells = np.zeros(raw.nelem, np.int32)
+    check(lib().fvx_mesh_lines(C.byref(d), C.c_double(threshold), C.byref(n),
+                               _ptr(ptr, C.c_int), _ptr(cells, C.c_int)))
+    return [cells[ptr[i]:ptr[i + 1]].copy() for i in range(n.value)]
+
+
+LINE_ORDERINGS = {"line": 0, "line_natural": 1, "line_rcm": 2}
+
+
+def line_permutation(raw: RawMesh, threshold: float, ordering: str = "line") -> np.ndarray:
+    """Cell order of "-mesh_reorder line | line_natural | line_rcm" (perm[new] = old)."""
+    d, keep = _desc(raw)
+    perm = np.zeros(raw.nelem, np.int32)
+    check(lib().fvx_mesh_line_order(C.byref(d), C.c_double(threshold),
+                                    LINE_ORDERINGS[ordering], _ptr(perm, C.c_int)))
+    return perm
+
+
 class Mesh:
     """Device mesh (UMesh after preprocessing) built by libfvx."""
 
@@ -333,7 +356,7 @@ PREC = {"f64": 0, "f32": 1}
 
 class TuneRow(C.Structure):
     _fields_ = [("kernel", C.c_int), ("block", C.c_int), ("xcd", C.c_int), ("ms", C.c_double)]
-PC = {"none": 0, "jacobi": 1, "sgs": 2, "ilu0": 3}
+PC = {"none": 0, "jacobi": 1, "sgs": 2, "ilu0": 3, "line": 4}
 
 
 class SolverCfg(C.Structure):
@@ -342,7 +365,7 @@ class SolverCfg(C.Structure):
                 ("min_relax", C.c_double), ("pc", C.c_int), ("pc_precision", C.c_int),
                 ("pc_sweeps", C.c_int), ("jac_precision", C.c_int), ("krylov_restart", C.c_int),
                 ("krylov_rtol", C.c_double), ("matrix_free", C.c_int), ("mf_eps", C.c_double),
-                ("fixed_norm", C.c_double)]
+                ("fixed_norm", C.c_double), ("line_threshold", C.c_double)]
 
 
 class SolveStats(C.Structure):
@@ -354,7 +377,7 @@ class SolveStats(C.Structure):
 def solver_cfg(cflinit=500.0, cflfin=5000.0, tol=1e-7, maxiter=170, implicit=True,
                fullupdate=False, min_relax=0.2, pc="sgs", pc_precision="f64", pc_sweeps=1,
                jac_precision="f64", krylov_restart=30, krylov_rtol=1e-1, matrix_free=False,
-               mf_eps=1e-7, fixed_norm=0.0):
+               mf_eps=1e-7, fixed_norm=0.0, line_threshold=10.0):
     """Defaults: testcases/naca0012/transonic-sanity-test-muscl.ctrl main solve with the
     linear-solver options of testcases/defaults.solverc."""
     c = SolverCfg()
@@ -364,6 +387,7 @@ def solver_cfg(cflinit=500.0, cflfin=5000.0, tol=1e-7, maxiter=170, implicit=Tru
     c.pc_sweeps = pc_sweeps
     c.krylov_restart, c.krylov_rtol = krylov_restart, krylov_rtol
     c.matrix_free, c.mf_eps, c.fixed_norm = int(matrix_free), mf_eps, fixed_norm
+    c.line_threshold = line_threshold
     return c
 
 
@@ -400,12 +424,17 @@ class Matrix:
 
 
 class Preconditioner:
-    """Device preconditioner (none / block Jacobi / multicolour SGS / multicolour BILU(0))."""
+    """Device preconditioner (none / block Jacobi / multicolour SGS / multicolour BILU(0) /
+    line Gauss-Seidel)."""
 
-    def __init__(self, mesh: Mesh, kind="ilu0", precision="f64", sweeps=1):
+    def __init__(self, mesh: Mesh, kind="ilu0", precision="f64", sweeps=1, line_threshold=10.0):
         self.mesh = mesh
         h = C.c_void_p()
-        check(lib().fvx_pc_create(mesh.h, PC[kind], PREC[precision], C.byref(h)))
+        if kind == "line":
+            check(lib().fvx_pc_create_lines(mesh.h, C.c_double(line_threshold), PREC[precision],
+                                            C.byref(h)))
+        else:
+            check(lib().fvx_pc_create(mesh.h, PC[kind], PREC[precision], C.byref(h)))
         self.h = h
         check(lib().fvx_pc_set_sweeps(h, int(sweeps)))
         self._A = None

@@ -141,6 +141,33 @@ struct MeshData {
 		check(fvx_mesh_rcm(&d, perm.data()));
 		permute_cells(perm);
 	}
+	/// "-mesh_reorder line | line_natural | line_rcm" with "-mesh_anisotropy_threshold"
+	/// (lineReorder / hybridLineReorder, src/mesh/ameshutils.cpp:56-82).
+	void reorder_lines(double threshold, const std::string& ordering = "line")
+	{
+		const int g = ordering == "line" ? FVX_LINES_ONLY
+		            : ordering == "line_natural" ? FVX_LINES_NATURAL
+		            : ordering == "line_rcm" ? FVX_LINES_RCM : -1;
+		if(g < 0) throw std::invalid_argument("unknown line ordering " + ordering);
+		std::vector<int> perm(nelem());
+		const fvx_mesh_desc d = desc();
+		check(fvx_mesh_line_order(&d, threshold, g, perm.data()));
+		permute_cells(perm);
+	}
+	/// preprocessMesh's "-mesh_reorder" (src/mesh/ameshutils.cpp:38-88): "natural" or empty
+	/// leaves the order, "rcm" and the line orderings reorder the cells.
+	void reorder(const std::string& ordering, double anisotropy_threshold = 0.0)
+	{
+		if(ordering.empty() || ordering == "natural") return;
+		if(ordering == "rcm") { reorder_rcm(); return; }
+		if(ordering.find("line") != std::string::npos) {
+			if(!(anisotropy_threshold > 0))
+				throw std::invalid_argument("No anisotropy threshold given!");
+			reorder_lines(anisotropy_threshold, ordering);
+			return;
+		}
+		throw std::invalid_argument("unsupported mesh ordering " + ordering);
+	}
 };
 
 /// Preprocessed device mesh (UMesh after compute_topological / compute_face_data).
@@ -293,7 +320,7 @@ struct SteadySolverConfig {
 	int maxiter = 170;
 	std::string nl_update_scheme = "ROBUST_FLOW";   ///< or "FULL"
 	double min_nl_update = 0.2;
-	std::string pc_type = "sgs";                     ///< none | jacobi | sgs | ilu0
+	std::string pc_type = "sgs";                     ///< none | jacobi | sgs | ilu0 | line
 	int pc_sweeps = 1;
 	int pc_precision = FVX_PREC_F64, jac_precision = FVX_PREC_F64;
 	int ksp_max_it = 30;
@@ -301,6 +328,7 @@ struct SteadySolverConfig {
 	bool matrix_free = false;
 	double matrix_free_difference_step = 1e-7;
 	double fixed_norm = 0.0;
+	double line_threshold = 10.0;                    ///< anisotropy threshold of "line"
 };
 
 struct TimingData {
@@ -324,7 +352,8 @@ public:
 		c.fullupdate = conf_.nl_update_scheme == "FULL";
 		c.min_relax = conf_.min_nl_update;
 		c.pc = detail::lookup(conf_.pc_type, {{"none", FVX_PC_NONE}, {"jacobi", FVX_PC_JACOBI},
-		                                      {"sgs", FVX_PC_SGS}, {"ilu0", FVX_PC_ILU0}},
+		                                      {"sgs", FVX_PC_SGS}, {"ilu0", FVX_PC_ILU0},
+		                                      {"line", FVX_PC_LINE}},
 		                      "preconditioner");
 		c.pc_precision = conf_.pc_precision;
 		c.pc_sweeps = conf_.pc_sweeps;
@@ -334,6 +363,7 @@ public:
 		c.matrix_free = conf_.matrix_free;
 		c.mf_eps = conf_.matrix_free_difference_step;
 		c.fixed_norm = conf_.fixed_norm;
+		c.line_threshold = conf_.line_threshold;
 		fvx_solve_stats st{};
 		const fvx_status s = fvx_steady_solve(space_->handle(), &c, u, &st, stream);
 		tdata_.ode_walltime = st.walltime;

@@ -15,10 +15,12 @@
  *   fvx_case FILE.ctrl [--mesh_file F] [--regression_file F] [--regression_tol 1e-8]
  *                      [--pc sgs] [--sweeps 16] [--precision f64|f32] [--matrix_free]
  *                      [--rcm] [--ksp_max_it 30] [--ksp_rtol 0.1] [--limiter_param 0]
+ *                      [--mesh_reorder rcm|line|line_natural|line_rcm]
+ *                      [--mesh_anisotropy_threshold T]
  *
- * The preconditioner, its sweeps and storage precision, matrix-free operation and the RCM
- * ordering stand in for the PETSc options file of the reference (-pc_type, -mesh_reorder,
- * -matrix_free_jacobian). The limiter parameter (WENO central weight, Venkatakrishnan K) is
+ * The preconditioner, its sweeps and storage precision, matrix-free operation and the cell
+ * ordering stand in for the PETSc options of the reference (-pc_type, -matrix_free_jacobian,
+ * -mesh_reorder with -mesh_anisotropy_threshold; --rcm is --mesh_reorder rcm). The limiter parameter (WENO central weight, Venkatakrishnan K) is
  * zero as in the reference's runs unless --limiter_param gives it.
  *
  * Flag mode (no control file) keeps the inviscid airfoil case of
@@ -88,6 +90,9 @@ int run_control_file(const std::string& ctrl, std::map<std::string, std::string>
 	const std::string meshfile = opt.count("mesh_file") ? opt["mesh_file"] : o.meshfile;
 	MeshData md = read_mesh(meshfile);
 	if(rcm) md.reorder_rcm();
+	else if(opt.count("mesh_reorder"))
+		md.reorder(opt["mesh_reorder"], opt.count("mesh_anisotropy_threshold")
+		                                ? std::stod(opt["mesh_anisotropy_threshold"]) : 0.0);
 	UMesh m(md);
 	std::printf("mesh %s: %d cells, %d faces\n", meshfile.c_str(), m.gnelem(), m.gnaface());
 

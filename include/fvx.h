@@ -85,6 +85,22 @@ typedef struct {
  * Replaces reorderMeshPetsc with "-mesh_reorder rcm" (src/mesh/ameshutils.cpp:245-288). */
 fvx_status fvx_mesh_rcm(const fvx_mesh_desc* desc, int* perm);
 
+/* Lines of cells whose neighbour-distance anisotropy exceeds threshold, grown from the
+ * boundary cells in boundary-face order (findLines, src/mesh/meshordering.cpp:199-262). Line l
+ * holds line_cells[line_ptr[l] .. line_ptr[l+1]); line_ptr needs nelem/2+1 entries at most,
+ * line_cells nelem. */
+fvx_status fvx_mesh_lines(const fvx_mesh_desc* desc, double threshold, int* nlines,
+                          int* line_ptr, int* line_cells);
+
+/* Line orderings, perm[new] = old: graph_ordering FVX_LINES_ONLY is "-mesh_reorder line"
+ * (lineReorder, meshordering.cpp:33-57); FVX_LINES_NATURAL and FVX_LINES_RCM are
+ * "-mesh_reorder line_natural" / "line_rcm" (getHybridLineOrdering, :68-128, with the
+ * line-point graph of :264-385 ordered by reverse Cuthill-McKee as PETSc's MatGetOrdering
+ * "rcm" does). */
+enum { FVX_LINES_ONLY = 0, FVX_LINES_NATURAL = 1, FVX_LINES_RCM = 2 };
+fvx_status fvx_mesh_line_order(const fvx_mesh_desc* desc, double threshold, int graph_ordering,
+                               int* perm);
+
 /* Builds topology (UMesh::compute_topological, src/mesh/mesh.cpp:330-340; face numbering of
  * compute_faceConnectivity :659-762), geometry (compute_areas, compute_face_data,
  * compute_cell_centres :290-365) and uploads the device mesh. */
@@ -144,7 +160,12 @@ enum { FVX_PREC_F64 = 0, FVX_PREC_F32 = 1 };
 /* Preconditioners (PETSc -pc_type none / pbjacobi / sor / bjacobi+ilu on one rank). SGS and
  * ILU0 run in a multicolour ordering of the cells (greedy colouring in cell order; cells of
  * one colour share no face), which makes each sweep of a colour fully parallel. */
-enum { FVX_PC_NONE = 0, FVX_PC_JACOBI = 1, FVX_PC_SGS = 2, FVX_PC_ILU0 = 3 };
+/* FVX_PC_LINE: symmetric line Gauss-Seidel. Its blocks are the lines of fvx_mesh_lines at the
+ * preconditioner's anisotropy threshold (default 10) and every other cell on its own, the
+ * vertices of the reference's line-point graph (src/mesh/meshordering.cpp:264-385); each line's
+ * block-tridiagonal system is solved exactly (block Thomas). The reference reaches the same
+ * effect through "-mesh_reorder line" followed by PETSc's ILU or SOR in that order. */
+enum { FVX_PC_NONE = 0, FVX_PC_JACOBI = 1, FVX_PC_SGS = 2, FVX_PC_ILU0 = 3, FVX_PC_LINE = 4 };
 
 typedef struct fvx_matrix_s* fvx_matrix;
 typedef struct fvx_pc_s* fvx_pc;
@@ -172,6 +193,8 @@ fvx_status fvx_matrix_export_bsr(fvx_matrix A, int* rowptr, int* colind, double*
 
 /* PCCreate/PCSetType for a matrix of this mesh; precision = storage of the factors. */
 fvx_status fvx_pc_create(fvx_mesh m, int type, int precision, fvx_pc* out);
+/* FVX_PC_LINE with the given anisotropy threshold of the lines. */
+fvx_status fvx_pc_create_lines(fvx_mesh m, double threshold, int precision, fvx_pc* out);
 fvx_status fvx_pc_destroy(fvx_pc p);
 /* Number of stationary iterations z <- z + M^-1 (r - A z) from z = 0 per application
  * (PETSc -pc_sor_its; for SGS, symmetric forward + backward sweeps). Default 1, at most 64. */
@@ -214,6 +237,7 @@ typedef struct {
 	int matrix_free;          /* 1: matrix-free operator, assembled Jacobian only in the PC */
 	double mf_eps;            /* matrix-free difference step */
 	double fixed_norm;        /* > 0: normalise residuals by this instead of the first norm */
+	double line_threshold;    /* FVX_PC_LINE: anisotropy threshold of the lines (<= 0: 10) */
 } fvx_solver_cfg;
 
 typedef struct {

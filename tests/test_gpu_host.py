@@ -94,3 +94,19 @@ def test_cpp_case_driver_runs_the_reference_weno_control_file(torch_cuda, tmp_pa
                        capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
     print(p.stdout, p.stderr)
     assert p.returncode == 0 and "REGRESSION PASSED" in p.stdout
+
+
+def test_cpp_case_driver_line_ordering_regression(torch_cuda, tmp_path):
+    # the reference's SpatialFlow_Euler_NACA0012_MUSCL_LS_HLLC_LineOrdering test:
+    # transonic-sanity-test-muscl.ctrl with -mesh_reorder line -mesh_anisotropy_threshold 10
+    # against regr-MUSCL_LS_HLLC_LineOrdering.txt (CDp to 1e-7, CL to 100x that). No cell of
+    # this mesh exceeds the threshold, so the order is the natural one, as the regression
+    # values (equal to the MUSCL ones to 1e-13) say
+    p = subprocess.run([DRIVER, golden(os.path.join("ctrl", "naca0012-sanity-muscl.ctrl")),
+                        "--mesh_file", golden("naca0012luo.msh"), "--mesh_reorder", "line",
+                        "--mesh_anisotropy_threshold", "10.0", "--regression_file",
+                        golden("naca0012_regr_MUSCL_LS_HLLC_LineOrdering.txt"),
+                        "--regression_tol", "1e-7"],
+                       capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
+    print(p.stdout, p.stderr)
+    assert p.returncode == 0 and "REGRESSION PASSED" in p.stdout
