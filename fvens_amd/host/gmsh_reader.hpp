@@ -1,15 +1,20 @@
-/* gmsh_reader.hpp -- Gmsh 2.2 ASCII reader for linear triangle / quad meshes.
+/* gmsh_reader.hpp -- Gmsh 2.2 ASCII reader and writer for linear triangle / quad meshes.
  *
  * Same reading rules as the reference reader (src/mesh/meshreaders.cpp:66-266): node
  * coordinates with z dropped, linear edges (type 1) are boundary faces whose first tag is the
  * physical marker, triangles (type 2) and quads (type 3) are cells, points (type 15) are
  * skipped; node numbers in the file are 1-based and must be contiguous.
+ *
+ * write_gmsh2 writes the layout of UMesh::writeGmsh2 (src/mesh/mesh.cpp:204-281): nodes with
+ * z = 0 at 20 significant digits, then the boundary faces (physical marker, elementary tag
+ * 1 + marker) and the cells (tags 1 1), numbered from 1 in that order.
  */
 #ifndef FVENS_GPU_GMSH_READER_HPP
 #define FVENS_GPU_GMSH_READER_HPP
 
 #include "fvens_gpu.hpp"
 #include <fstream>
+#include <iomanip>
 #include <sstream>
 
 namespace fvens_gpu {
@@ -69,6 +74,27 @@ inline MeshData read_gmsh2(const std::string& path)
 	}
 	if(!have_nodes) throw std::runtime_error("no $Nodes section in " + path);
 	return m;
+}
+
+inline void write_gmsh2(const std::string& path, const MeshData& m)
+{
+	std::ofstream out(path);
+	if(!out) throw std::runtime_error("cannot write " + path);
+	out << std::setprecision(20);
+	out << "$MeshFormat\n2.2 0 8\n$EndMeshFormat\n$Nodes\n" << m.npoin() << '\n';
+	for(int p = 0; p < m.npoin(); p++)
+		out << p + 1 << ' ' << m.coords[2*(size_t)p] << ' ' << m.coords[2*(size_t)p + 1] << ' '
+		    << 0.0 << '\n';
+	out << "$EndNodes\n$Elements\n" << m.nbface() + m.nelem() << '\n';
+	for(int b = 0; b < m.nbface(); b++)
+		out << b + 1 << " 1 2 " << m.btag[b] << ' ' << 1 + m.btag[b] << ' '
+		    << m.bface[2*(size_t)b] + 1 << ' ' << m.bface[2*(size_t)b + 1] + 1 << '\n';
+	for(int e = 0; e < m.nelem(); e++) {
+		out << m.nbface() + e + 1 << ' ' << (m.nnode[e] == 3 ? 2 : 3) << " 2 1 1";
+		for(int k = 0; k < m.nnode[e]; k++) out << ' ' << m.inpoel[4*(size_t)e + k] + 1;
+		out << '\n';
+	}
+	out << "$EndElements\n";
 }
 
 }

@@ -82,6 +82,11 @@ int main(int argc, char** argv)
 			write_vtu_point_data(argv[3], m, postprocess_point(m, u, area, 1.4, 0.5));
 			return 0;
 		}
+		if(!std::strcmp(argv[1], "msh") && argc > 3) {
+			// any readable mesh written back as Gmsh 2.2
+			write_gmsh2(argv[3], read_mesh(argv[2]));
+			return 0;
+		}
 		if(!std::strcmp(argv[1], "surf") && argc > 4) {
 			// surface file of the "other" marker argv[4] under prefix argv[3], for the state
 			// rho = 1 + 0.001 (e mod 17), velocity (0.5, 0.01 (e mod 5))

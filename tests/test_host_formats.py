@@ -154,3 +154,21 @@ def test_other_marker_surface_file(harness, tmp_path):
     e = host[far]
     np.testing.assert_allclose(rows[:, 2], 0.5, rtol=1e-12)
     np.testing.assert_allclose(rows[:, 3], 0.01 * (e % 5), rtol=1e-5, atol=1e-12)
+
+
+def test_gmsh_writer_round_trip(harness, tmp_path):
+    # UMesh::writeGmsh2 layout (src/mesh/mesh.cpp:204-281), used by the reference's format
+    # converter: the SU2 mesh written as Gmsh 2.2 reads back as the same mesh, with the
+    # reference's element numbering and tags
+    out = str(tmp_path / "naca.msh")
+    subprocess.run([harness, "msh", golden("NACA0012_lam_hybrid_1.su2"), out], check=True)
+    a = su2.read_su2(golden("NACA0012_lam_hybrid_1.su2"))
+    b = gmsh.read_gmsh2(out)
+    assert np.array_equal(a.coords, b.coords)
+    assert np.array_equal(a.inpoel, b.inpoel) and np.array_equal(a.nnode, b.nnode)
+    assert np.array_equal(a.bface, b.bface) and np.array_equal(a.btag, b.btag)
+    lines = open(out).read().splitlines()
+    first = lines.index("$Elements") + 2
+    assert lines[first].split()[:5] == ["1", "1", "2", str(a.btag[0]), str(1 + a.btag[0])]
+    cell = lines[first + a.nbface].split()
+    assert cell[0] == str(a.nbface + 1) and cell[2:5] == ["2", "1", "1"]
