@@ -129,6 +129,34 @@ def gaussian_bump(nx: int = 96, ny: int = 32, nsplit: int = 16) -> RawMesh:
     return _orient_ccw(m)
 
 
+def gaussian_channel(nx: int, ny: int, xend: float = 1.0, height: float = 0.75, a: float = 0.02,
+                     b: float = 100.0) -> RawMesh:
+    """All-triangle channel x in [-xend, xend] between y = a exp(-b x^2) and y = height: the
+    geometry and markers of the reference's entropy-convergence case
+    (tests/inv-gaussianbump/gaussian_channel.geo: walls 2, inlet 3, outlet 4), meshed here as
+    nx*ny quads each split into two triangles instead of Gmsh's unstructured triangles."""
+    xs = np.linspace(-xend, xend, nx + 1)
+    ylow = a * np.exp(-b * xs ** 2)
+    t = np.arange(ny + 1) / ny
+    X = np.repeat(xs[:, None], ny + 1, axis=1)
+    Y = ylow[:, None] + (height - ylow[:, None]) * t[None, :]
+    nid = np.arange((nx + 1) * (ny + 1), dtype=np.int32).reshape(nx + 1, ny + 1)
+    coords = np.stack([X.ravel(), Y.ravel()], axis=1)
+    inpoel, nnode = _cells_from_grid(nid, ny, True)
+    bf, bt = [], []
+    for i in range(nx):
+        bf.append((nid[i, 0], nid[i + 1, 0])); bt.append(2)
+    for i in range(nx):
+        bf.append((nid[i + 1, ny], nid[i, ny])); bt.append(2)
+    for k in range(ny):
+        bf.append((nid[0, k + 1], nid[0, k])); bt.append(3)
+    for k in range(ny):
+        bf.append((nid[nx, k], nid[nx, k + 1])); bt.append(4)
+    m = RawMesh(coords, inpoel, nnode, np.array(bf, np.int32), np.array(bt, np.int32),
+                f"channel{nx}x{ny}")
+    return _orient_ccw(m)
+
+
 def naca4_thickness(x: np.ndarray, t: float = 0.12) -> np.ndarray:
     return 5.0 * t * (0.2969 * np.sqrt(x) - 0.1260 * x - 0.3516 * x ** 2 + 0.2843 * x ** 3
                       - 0.1036 * x ** 4)
