@@ -88,6 +88,7 @@ struct fvx_mesh_s {
 	// device copies
 	fvx::DBuf<int> d_nbr, d_face;          // [nelem][4] slots: neighbour / face index
 	fvx::DBuf<fvx::FaceGeom> d_fgeom;      // [naface]
+	fvx::DBuf<double> d_soff;              // [nelem][4][2] face midpoint minus cell centre
 	fvx::DBuf<double> d_centre;            // [nelem][2]
 	fvx::DBuf<double> d_area;              // [nelem]
 	fvx::DBuf<int> d_bhost, d_btag;        // [nbface] host cell and marker of boundary faces
@@ -191,8 +192,8 @@ struct fvx_pc_s {
 	// coff, their cells, and per cell the slots of its line neighbours
 	double line_threshold = 10.0;
 	int nvert = 0;
-	std::vector<int> vptr, vcells;
-	fvx::DBuf<int> d_vptr, d_vcells;
+	std::vector<int> vptr, vcells, lcnt;   // lcnt: lines among the vertices of each colour
+	fvx::DBuf<int> d_vptr, d_vcells, d_ocell;   // d_ocell: first cell of each vertex in order
 	fvx::DBuf<unsigned char> d_lineinfo;
 	fvx::DBuf<double> v64;
 	fvx::DBuf<float> v32;

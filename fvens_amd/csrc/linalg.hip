@@ -828,20 +828,34 @@ static void line_passes(const fvx_pc_s& p, const VTF* fac, const double* r, long
 {
 	const int* nbr = p.m->d_nbr.p;
 	const long sa = mstride(p.m);
+	// per colour: the lines (one thread each), then the lone cells (one quad each; their factor
+	// blocks are the point Gauss-Seidel ones: inverted diagonal, matrix off-diagonals)
 	auto colour = [&](int c, bool all) {
-		const int cnt = p.coff[c+1] - p.coff[c];
-		if(cnt == 0) return;
-		const int* list = p.d_order.p + p.coff[c];
-		const dim3 g(thread_blocks(cnt), L.nact);
-		if(all)
-			hipLaunchKernelGGL((k_line_colour<VTF, true>), g, dim3(BLOCK), 0, s, cnt, list,
-			                   p.d_vptr.p, p.d_vcells.p, p.d_lineinfo.p, L.list, fac, sa, nbr,
-			                   p.d_lowmask.p, r, sr, z, sz);
-		else
-			hipLaunchKernelGGL((k_line_colour<VTF, false>), g, dim3(BLOCK), 0, s, cnt, list,
-			                   p.d_vptr.p, p.d_vcells.p, p.d_lineinfo.p, L.list, fac, sa, nbr,
-			                   p.d_lowmask.p, r, sr, z, sz);
-		FVX_HIP(hipGetLastError());
+		const int nl = p.lcnt[c], np = p.coff[c+1] - p.coff[c] - nl;
+		if(nl > 0) {
+			const int* list = p.d_order.p + p.coff[c];
+			const dim3 g(thread_blocks(nl), L.nact);
+			if(all)
+				hipLaunchKernelGGL((k_line_colour<VTF, true>), g, dim3(BLOCK), 0, s, nl, list,
+				                   p.d_vptr.p, p.d_vcells.p, p.d_lineinfo.p, L.list, fac, sa, nbr,
+				                   p.d_lowmask.p, r, sr, z, sz);
+			else
+				hipLaunchKernelGGL((k_line_colour<VTF, false>), g, dim3(BLOCK), 0, s, nl, list,
+				                   p.d_vptr.p, p.d_vcells.p, p.d_lineinfo.p, L.list, fac, sa, nbr,
+				                   p.d_lowmask.p, r, sr, z, sz);
+			FVX_HIP(hipGetLastError());
+		}
+		if(np > 0) {
+			const int* cells = p.d_ocell.p + p.coff[c] + nl;
+			const dim3 g(quad_blocks(np), L.nact);
+			if(all)
+				hipLaunchKernelGGL((k_sgs_colour<VTF, true>), g, dim3(BLOCK), 0, s, np, cells,
+				                   L.list, fac, sa, nbr, p.d_lowmask.p, r, sr, z, sz);
+			else
+				hipLaunchKernelGGL((k_sgs_colour<VTF, false>), g, dim3(BLOCK), 0, s, np, cells,
+				                   L.list, fac, sa, nbr, p.d_lowmask.p, r, sr, z, sz);
+			FVX_HIP(hipGetLastError());
+		}
 	};
 	for(int c = 0; c < p.ncol; c++) colour(c, !first);
 	for(int c = p.ncol - 1; c >= 0; c--) colour(c, true);

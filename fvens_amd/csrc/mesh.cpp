@@ -205,9 +205,20 @@ void build_mesh(fvx_mesh_s& m, const fvx_mesh_desc& d)
 	for(int i = 0; i < m.naface; i++)
 		fg[i] = FaceGeom{m.facemetric[(size_t)i*3], m.facemetric[(size_t)i*3+1],
 		                 m.facemetric[(size_t)i*3+2], m.gr[(size_t)i*2], m.gr[(size_t)i*2+1], 0.0};
+	// per slot, the offset of the face midpoint from the cell centre, contiguous per cell (the
+	// limiter's extrapolation distances; the same differences it would form from the face
+	// records, without their scattered reads)
+	std::vector<double> soff((size_t)m.nelem*8, 0.0);
+	for(int c = 0; c < m.nelem; c++)
+		for(int j = 0; j < m.nnode[c]; j++) {
+			const FaceGeom& G = fg[fidx[(size_t)c*4+j]];
+			soff[(size_t)c*8+2*j] = G.mx - m.rc[c*2];
+			soff[(size_t)c*8+2*j+1] = G.my - m.rc[c*2+1];
+		}
 	m.d_nbr.upload(nbr);
 	m.d_face.upload(fidx);
 	m.d_fgeom.upload(fg);
+	m.d_soff.upload(soff);
 	m.d_centre.upload(m.rc);
 	m.d_area.upload(m.area);
 	m.d_bhost.upload(host);

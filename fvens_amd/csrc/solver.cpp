@@ -141,9 +141,17 @@ static void line_vertices(fvx_pc_s& p, const fvx_mesh_s& m)
 	p.coff.assign(p.ncol + 1, 0);
 	for(int v = 0; v < p.nvert; v++) p.coff[vcol[v] + 1]++;
 	for(int k = 0; k < p.ncol; k++) p.coff[k+1] += p.coff[k];
+	// within a colour the lines come first (they have the lower vertex numbers); the lone cells
+	// after them are swept by the point Gauss-Seidel kernel through their cell numbers
 	p.order.assign(p.nvert, 0);
-	std::vector<int> fill(p.coff.begin(), p.coff.end() - 1);
-	for(int v = 0; v < p.nvert; v++) p.order[fill[vcol[v]]++] = v;
+	std::vector<int> fill(p.coff.begin(), p.coff.end() - 1), ocell(p.nvert);
+	p.lcnt.assign(p.ncol, 0);
+	for(int v = 0; v < p.nvert; v++) {
+		p.order[fill[vcol[v]]++] = v;
+		if(v < (int)lines.size()) p.lcnt[vcol[v]]++;
+	}
+	for(int k = 0; k < p.nvert; k++) ocell[k] = p.vcells[p.vptr[p.order[k]]];
+	p.d_ocell.upload(ocell);
 	p.d_vptr.upload(p.vptr);
 	p.d_vcells.upload(p.vcells);
 	p.d_lineinfo.upload(info);

@@ -33,7 +33,7 @@ template <int LIM, bool CONS>
 __global__ __launch_bounds__(MAX_BLOCK)
 void k_gradients(int ncell, const double* __restrict__ u, const int* __restrict__ nbr,
                  const int* __restrict__ face, const FaceGeom* __restrict__ fgeom,
-                 const double* __restrict__ centre, const double* __restrict__ gcoef,
+                 const double* __restrict__ soff, const double* __restrict__ gcoef,
                  const double* __restrict__ eps2, const int* __restrict__ bcid,
                  const GasParams* __restrict__ gpa, const BCTable* __restrict__ bta,
                  const BatchArg ba, double* __restrict__ gout, double* __restrict__ rout,
@@ -67,14 +67,12 @@ void k_gradients(int ncell, const double* __restrict__ u, const int* __restrict_
 #pragma unroll
 	for(int k = 0; k < 2; k++) load4(u + 4*(nbs[k] >= 0 ? nbs[k] : cc), uj[k]);
 	const double2 c0 = cf[0], c1 = cf[1];
-	double fx[2] = {0, 0}, fy[2] = {0, 0};
+	// face midpoint minus cell centre of the two slots (zero for an unused slot)
+	double2 off[2] = { make_double2(0, 0), make_double2(0, 0) };
 	if constexpr(LIM > 0) {
-#pragma unroll
-		for(int k = 0; k < 2; k++) {
-			if(nbs[k] == SLOT_UNUSED) continue;
-			const double2 m = *reinterpret_cast<const double2*>(&fgeom[fcs[k]].mx);
-			fx[k] = m.x; fy[k] = m.y;
-		}
+		const double2* so = reinterpret_cast<const double2*>(soff) + 4*cc + 2*p;
+		off[0] = so[0];
+		off[1] = so[1];
 	}
 #pragma unroll
 	for(int k = 0; k < 2; k++) {
@@ -107,7 +105,6 @@ void k_gradients(int ncell, const double* __restrict__ u, const int* __restrict_
 	if constexpr(LIM == 0) {
 		if(valid) store4(gout + 8*cc + 4*p, g + 4*p);
 	} else {
-		const double xi = centre[2*cc], yi = centre[2*cc+1];
 		const double e2 = LIM == 1 ? eps2[cc] : 0.0;
 		double lg[8];
 		for(int v = 0; v < 4; v++) {
@@ -117,8 +114,8 @@ void k_gradients(int ncell, const double* __restrict__ u, const int* __restrict_
 				if(nbs[k] == SLOT_UNUSED) continue;
 				// face value by linear extrapolation, then its difference from the cell value
 				double uf = wi[v];
-				uf += g[2*v]*(fx[k] - xi);
-				uf += g[2*v+1]*(fy[k] - yi);
+				uf += g[2*v]*off[k].x;
+				uf += g[2*v+1]*off[k].y;
 				const double dm = uf - wi[v];
 				double ph;
 				if constexpr(LIM == 1) {
@@ -248,7 +245,7 @@ void launch_gradients(const fvx_disc_s& d, const double* u, double* grad, double
 	const int lim = conserved ? 0 : (d.nc.reconstruction == FVX_RECON_VENKATAKRISHNAN ? 1 :
 	                                 d.nc.reconstruction == FVX_RECON_BARTHJESPERSEN ? 2 : 0);
 #define FVX_GL(L, C) hipLaunchKernelGGL((k_gradients<L, C>), dim3(nblk, nact), dim3(block), 0, \
-		s, m.nelem, u, m.d_nbr.p, m.d_face.p, m.d_fgeom.p, m.d_centre.p, d.d_gcoef.p, \
+		s, m.nelem, u, m.d_nbr.p, m.d_face.p, m.d_fgeom.p, m.d_soff.p, d.d_gcoef.p, \
 		d.d_eps2.p, d.d_bcid.p, ba.gp ? ba.gp : d.d_gas.p, d.d_bct.p, ba, grad, rgrad, \
 		d.tune.grad_xcd)
 	d.prof.start(PROF_GRADIENTS, s);
