@@ -2,8 +2,8 @@
  * the converged state of an operating point, trained and evaluated by the kernels below.
  *
  * No reference counterpart: the reference starts a solve from the freestream or from a
- * first-order starter solve (src/utilities/casesolvers.cpp:225-386). The implicit solver stays
- * the judge: a predicted state is only an initial guess.
+ * first-order starter solve (src/utilities/casesolvers.cpp:225-386). The implicit solver still
+ * decides the result: a predicted state is only an initial guess.
  *
  * Network (width 64, 4 layers; relu = max(0, .); bf16 weights, fp32 accumulation):
  *   h0 = x Wenc + benc                                  x: 8 node features
