@@ -113,3 +113,29 @@ def test_cpp_case_driver_line_ordering_regression(torch_cuda, tmp_path):
                        capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
     print(p.stdout, p.stderr)
     assert p.returncode == 0 and "REGRESSION PASSED" in p.stdout
+
+
+@pytest.mark.parametrize("ordering,ctrl,regr", [
+    ("line", "visc-naca0012-laminar-implicit.ctrl", "visc_naca0012_regr_LS_Roe_LineOrdering.txt"),
+    ("line_rcm", "visc-naca0012-laminar-implicit-linercm.ctrl",
+     "visc_naca0012_regr_LS_Roe_LineOrdering_RCM.txt")])
+def test_cpp_case_driver_viscous_line_orderings(torch_cuda, tmp_path, ordering, ctrl, regr):
+    # the reference's SpatialFlow_NS_NACA0012_LS_Roe_LineOrdering(_RCM) tests: the laminar case
+    # on the mesh reordered with -mesh_reorder line / line_rcm, threshold 10 (198 lines). The
+    # three regression files of this case stop at 1e-6 on different solver paths and differ
+    # among themselves by 3e-7 in CDp, 2e-7 in CDsf and 2.3e-7 absolute in CL (7e-3 of it);
+    # the device solve on the reordered mesh is held to 1e-6 relative in CDp and CDsf and
+    # 5e-7 absolute in CL
+    p = subprocess.run([DRIVER, golden(os.path.join("ctrl", ctrl)),
+                        "--mesh_file", golden("NACA0012_lam_hybrid_1.msh"),
+                        "--mesh_reorder", ordering, "--mesh_anisotropy_threshold", "10.0"],
+                       capture_output=True, text=True, timeout=120, cwd=str(tmp_path))
+    print(p.stdout, p.stderr)
+    assert p.returncode == 0
+    line = [ln for ln in p.stdout.splitlines() if ln.startswith("CL ")][0].split()
+    cl, cdp, cdsf = float(line[1]), float(line[3]), float(line[5])
+    with open(golden(regr)) as fh:
+        CL, CDp, CDsf = (float(t) for t in fh.read().split()[:3])
+    assert abs(cl - CL) <= 5e-7, (cl, CL)
+    assert abs(cdp - CDp) <= 1e-6 * abs(CDp), (cdp, CDp)
+    assert abs(cdsf - CDsf) <= 1e-6 * abs(CDsf), (cdsf, CDsf)
