@@ -14,6 +14,7 @@
  *
  *   fvx_case FILE.ctrl [--mesh_file F] [--regression_file F] [--regression_tol 1e-8]
  *                      [--pc sgs] [--sweeps 16] [--precision f64|f32] [--matrix_free]
+ *                      [--matrix_free_difference_step 1e-7]
  *                      [--rcm] [--ksp_max_it 30] [--ksp_rtol 0.1] [--limiter_param 0]
  *                      [--mesh_reorder rcm|line|line_natural|line_rcm]
  *                      [--mesh_anisotropy_threshold T]
@@ -127,6 +128,8 @@ int run_control_file(const std::string& ctrl, std::map<std::string, std::string>
 	sc.cflinit = o.initcfl; sc.cflfin = o.endcfl;
 	sc.tol = o.tolerance; sc.maxiter = o.maxiter;
 	sc.matrix_free = mf;
+	if(opt.count("matrix_free_difference_step"))
+		sc.matrix_free_difference_step = std::stod(opt["matrix_free_difference_step"]);
 	SteadySolver solver(&prob, sc, implicit);
 	try { solver.solve(u.p); }
 	catch(const Tolerance_error& e) { std::printf("main: %s\n", e.what()); }
