@@ -125,7 +125,7 @@ def test_cpp_case_driver_viscous_line_orderings(torch_cuda, tmp_path, ordering, 
     # three regression files of this case stop at 1e-6 on different solver paths and differ
     # among themselves by 3e-7 in CDp, 2e-7 in CDsf and 2.3e-7 absolute in CL (7e-3 of it).
     # The device solve on the reordered mesh also stops at 1e-6, on its own path, which leaves
-    # CDp 3e-6 to 5e-6 below the files (the converged value agrees with all three to 1e-7), so
+    # CDp 3e-6 to 5e-6 below the files (the converged value agrees with all three to 2e-7), so
     # drag is held to 1e-5 relative as in the unordered test and CL to 5e-7 absolute
     p = subprocess.run([DRIVER, golden(os.path.join("ctrl", ctrl)),
                         "--mesh_file", golden("NACA0012_lam_hybrid_1.msh"),
