@@ -489,3 +489,30 @@ def test_entropy_error_converges_at_second_order(torch_cuda, num):
     assert 1.65 <= slopes[-1] <= 2.3, slopes
     eo = _channel_entropy(0, num, 1e-9, oracle=True)
     assert abs(res[0][1] - eo) <= 1e-8 * eo, (res[0][1], eo)
+
+
+def test_viscous_naca0012_with_line_gauss_seidel(torch_cuda):
+    # the reference's laminar case solved with the line preconditioner (198 lines at threshold
+    # 10, FP32 factors, 4 sweeps) converges to 1e-10 and to the same discrete solution as the
+    # point-SGS solve (CL to 1e-10 absolute, drag to 1e-8)
+    import torch
+    import test_oracle_solver as tos
+    raw = gmsh.read_gmsh2(golden("NACA0012_lam_hybrid_1.msh"))
+    gm = fvx.Mesh(raw)
+    d1 = fvx.FlowFV(gm, fvx.physics_cfg(**tos.VISC_NACA_PHYS),
+                    fvx.numerics_cfg(flux="ROE", gradient="LEASTSQUARES", order2=False))
+    d2 = fvx.FlowFV(gm, fvx.physics_cfg(**tos.VISC_NACA_PHYS),
+                    fvx.numerics_cfg(flux="ROE", gradient="LEASTSQUARES", recon="NONE"))
+    res = {}
+    for name, pc in (("sgs", dict(pc="sgs", pc_sweeps=16)),
+                     ("line", dict(pc="line", pc_sweeps=4, line_threshold=10.0))):
+        u = torch.tensor(np.tile(d1.freestream(), (raw.nelem, 1)), device="cuda")
+        kw = dict(pc_precision="f32", jac_precision="f32", min_relax=0.2, **pc)
+        d1.steady_solve(u, raise_on_failure=False, cflinit=200.0, cflfin=1000.0, tol=1e-1,
+                        maxiter=50, **kw)
+        s2 = d2.steady_solve(u, cflinit=500.0, cflfin=5000.0, tol=1e-10, maxiter=400, **kw)
+        assert s2["converged"], (name, s2)
+        res[name] = d2.functionals(u, 2)
+    assert abs(res["line"]["CL"] - res["sgs"]["CL"]) <= 1e-10
+    for k in ("CDp", "CDsf"):
+        assert abs(res["line"][k] - res["sgs"][k]) <= 1e-8 * abs(res["sgs"][k]), k
