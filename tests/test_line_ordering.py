@@ -61,3 +61,17 @@ def test_cell_rcm_on_a_strip_is_a_band_of_width_one():
     assert sorted(perm) == list(range(n))
     pos = shuffle[perm]            # strip position of each new cell
     assert np.all(np.abs(np.diff(pos)) == 1)
+
+
+def test_line_ordering_rejects_bad_arguments(aniso):
+    import ctypes as C
+    d, keep = fvx._desc(aniso)
+    perm = np.zeros(aniso.nelem, np.int32)
+    rc = fvx.lib().fvx_mesh_line_order(C.byref(d), C.c_double(10.0), 7,
+                                       perm.ctypes.data_as(C.POINTER(C.c_int)))
+    assert rc != 0 and b"line-graph ordering" in fvx.lib().fvx_last_error()
+    n = C.c_int()
+    rc = fvx.lib().fvx_mesh_lines(C.byref(d), C.c_double(10.0), C.byref(n), None, None)
+    assert rc != 0
+    with pytest.raises(KeyError):
+        fvx.line_permutation(aniso, 10.0, "line_amd")
