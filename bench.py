@@ -291,6 +291,8 @@ def warm_start_ab(params_path, ntest, nsingle, maxiter, tol=1e-8, sweeps=16):
         "batched": {"plain_fp64_seconds": round(ab["t_cold"], 3),
                     "warm_fp32_seconds": round(ab["t_warm"] + ab["t_fallback"], 3),
                     "speedup": round(ab["speedup_batched"], 3), "fallbacks": ab["fallbacks"],
+                    "warm_phases_seconds": {k: round(ab[k], 3) for k in
+                                            ("t_inference", "t_pre", "t_fallback")},
                     "plain_steps": [p["cold_steps"] for p in pts],
                     "warm_steps": [p["warm_steps"] for p in pts],
                     "both_converged": len(ok),
