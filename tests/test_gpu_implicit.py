@@ -516,3 +516,41 @@ def test_viscous_naca0012_with_line_gauss_seidel(torch_cuda):
     assert abs(res["line"]["CL"] - res["sgs"]["CL"]) <= 1e-10
     for k in ("CDp", "CDsf"):
         assert abs(res["line"][k] - res["sgs"][k]) <= 1e-8 * abs(res["sgs"][k]), k
+
+
+@pytest.mark.parametrize("grad", ["LEASTSQUARES", "GREENGAUSS"])
+def test_cylinder_entropy_convergence_on_the_device(torch_cuda, grad):
+    # the reference's cylinder entropy-convergence tests (tests/inv-2dcyl) run on the device with
+    # the control files' settings: slope between the two finest of the reference's meshes in
+    # [1.65, 2.1]; converged to 1e-9 on the 512-cell mesh, the entropy error equals the oracle's
+    # to 1e-8
+    import torch
+    import test_oracle_solver as tos
+
+    def device(level, tol):
+        raw = gmsh.read_gmsh2(golden(f"2dcylinder{level}.msh"))
+        gm = fvx.Mesh(raw)
+        phys = fvx.physics_cfg(Minf=0.38, bcs=tos.CYL_BCS)
+        d1 = fvx.FlowFV(gm, phys, fvx.numerics_cfg(flux="HLLC", gradient="NONE", order2=False))
+        d2 = fvx.FlowFV(gm, phys, fvx.numerics_cfg(flux="HLLC", gradient=grad, recon="NONE"))
+        u = torch.tensor(np.tile(d1.freestream(), (raw.nelem, 1)), device="cuda")
+        sch = tos.CYL_SCHEMES[grad]
+        kw = dict(pc="sgs", pc_sweeps=4, min_relax=0.2)
+        c0, c1, n1 = sch["starter"]
+        d1.steady_solve(u, raise_on_failure=False, cflinit=c0, cflfin=c1, tol=1e-1, maxiter=n1,
+                        **kw)
+        c0, c1, n2 = sch["main"]
+        s2 = d2.steady_solve(u, cflinit=c0, cflfin=c1, tol=tol, maxiter=max(n2, 400), **kw)
+        assert s2["converged"]
+        return raw.nelem, d2.functionals(u, 2)["entropy"]
+
+    res = [device(k, 1e-5) for k in range(4)]
+    lh = [np.log10(1.0 / np.sqrt(n)) for n, _ in res]
+    le = [np.log10(e) for _, e in res]
+    slope = (le[3] - le[2]) / (lh[3] - lh[2])
+    print(grad, "entropy errors", [e for _, e in res], "slope", slope)
+    assert 1.65 <= slope <= 2.1, slope
+    _, eo, so = tos.cylinder_entropy(1, grad, tol=1e-9)
+    _, eg = device(1, 1e-9)
+    assert so["converged"]
+    assert abs(eg - eo) <= 1e-8 * eo, (eg, eo)

@@ -329,3 +329,49 @@ def test_viscous_naca0012_reference_regression():
     assert abs(f["CL"] - CL) <= 5e-7
     assert abs(f["CDp"] - CDp) / abs(CDp) <= 2e-7
     assert abs(f["CDsf"] - CDsf) / abs(CDsf) <= 2e-7
+
+
+# ------------------------------------------------------------- entropy convergence ---------
+
+CYL_BCS = [(2, "slipwall"), (4, "farfield")]
+CYL_SCHEMES = {
+    # tests/inv-2dcyl/inv-cyl-ls-hllc.ctrl and inv-cyl-gg-hllc_tri.ctrl: starter CFL ramp,
+    # main CFL ramp, main tolerance and step limit
+    "LEASTSQUARES": dict(starter=(25.0, 500.0, 150), main=(250.0, 5000.0, 300)),
+    "GREENGAUSS": dict(starter=(25.0, 250.0, 250), main=(250.0, 1000.0, 350)),
+}
+
+
+def cylinder_entropy(level, grad, tol=1e-5):
+    """Entropy error of the M0.38 cylinder (tests/inv-2dcyl/inv-cyl-base.ctrl, HLLC, unlimited)
+    on the reference's mesh testcases/2dcylinder/grids/2dcylinder<level>.msh, solved with the
+    control file's starter and main settings; returns (cells, entropy error, main stats)."""
+    raw = gmsh.read_gmsh2(golden(f"2dcylinder{level}.msh"))
+    m = po.Mesh(raw)
+    sch = CYL_SCHEMES[grad]
+    d1 = po.Disc(m, po.make_config(Minf=0.38, bcs=CYL_BCS, flux="HLLC", gradient="NONE",
+                                   order2=False))
+    d2 = po.Disc(m, po.make_config(Minf=0.38, bcs=CYL_BCS, flux="HLLC", gradient=grad,
+                                   recon="NONE"))
+    u = np.tile(d1.freestream(), (raw.nelem, 1))
+    c0, c1, n1 = sch["starter"]
+    u, _ = d1.steady_solve(u, cflinit=c0, cflfin=c1, tol=1e-1, maxiter=n1, fullupdate=0,
+                           min_relax=0.2, pc="ilu0")
+    c0, c1, n2 = sch["main"]
+    u, s2 = d2.steady_solve(u, cflinit=c0, cflfin=c1, tol=tol, maxiter=n2, fullupdate=0,
+                            min_relax=0.2, pc="ilu0")
+    return raw.nelem, d2.functionals(u, 2)["entropy"], s2
+
+
+@pytest.mark.parametrize("grad", ["LEASTSQUARES", "GREENGAUSS"])
+def test_cylinder_entropy_convergence_reference_criterion(grad):
+    # SpatialFlow_Euler_Cylinder_{LeastSquares,GreenGauss}_HLLC_Tri_EntropyConvergence
+    # (tests/inv-2dcyl/CMakeLists.txt, tests/flow_conv.cpp:78-89): on the reference's four
+    # cylinder meshes, h = 1/sqrt(cells), the slope of log(entropy error) against log(h) between
+    # the two finest meshes lies in [1.65, 2.1]
+    res = [cylinder_entropy(k, grad) for k in range(4)]
+    assert all(s["converged"] for _, _, s in res)
+    lh = [np.log10(1.0 / np.sqrt(n)) for n, _, _ in res]
+    le = [np.log10(e) for _, e, _ in res]
+    slope = (le[3] - le[2]) / (lh[3] - lh[2])
+    assert 1.65 <= slope <= 2.1, slope
