@@ -556,3 +556,32 @@ def test_cylinder_entropy_convergence_on_the_device(torch_cuda, grad):
     _, eg = device(1, 1e-9)
     assert so["converged"]
     assert abs(eg - eo) <= 1e-8 * eo, (eg, eo)
+
+
+def test_explicit_cylinder_entropy_convergence_on_the_device(torch_cuda):
+    # the reference's Flow_Explicit_Euler_Cylinder_GreenGauss_Roe_Tri_EntropyConvergence test
+    # (tests/inv-2dcyl/expl-inv-cyl-gg-roe_tri.ctrl, three meshes): forward-Euler pseudo-time
+    # stepping (starter CFL 0.5 -> 0.7 to 1e-1, main CFL 0.25 -> 0.3 to 1e-4), Roe, Green-Gauss,
+    # unlimited; slope between the two finest meshes in [1.65, 2.1]
+    import torch
+    import test_oracle_solver as tos
+    res = []
+    for level in range(3):
+        raw = gmsh.read_gmsh2(golden(f"2dcylinder{level}.msh"))
+        gm = fvx.Mesh(raw)
+        phys = fvx.physics_cfg(Minf=0.38, bcs=tos.CYL_BCS)
+        d1 = fvx.FlowFV(gm, phys, fvx.numerics_cfg(flux="ROE", gradient="NONE", order2=False))
+        d2 = fvx.FlowFV(gm, phys, fvx.numerics_cfg(flux="ROE", gradient="GREENGAUSS",
+                                                   recon="NONE"))
+        u = torch.tensor(np.tile(d1.freestream(), (raw.nelem, 1)), device="cuda")
+        d1.steady_solve(u, raise_on_failure=False, implicit=False, cflinit=0.5, cflfin=0.7,
+                        tol=1e-1, maxiter=15000)
+        s2 = d2.steady_solve(u, implicit=False, cflinit=0.25, cflfin=0.3, tol=1e-4,
+                             maxiter=60000)
+        assert s2["converged"]
+        res.append((raw.nelem, d2.functionals(u, 2)["entropy"], s2["steps"]))
+    lh = [np.log10(1.0 / np.sqrt(n)) for n, _, _ in res]
+    le = [np.log10(e) for _, e, _ in res]
+    slope = (le[2] - le[1]) / (lh[2] - lh[1])
+    print("explicit entropy errors", res, "slope", slope)
+    assert 1.65 <= slope <= 2.1, slope
