@@ -62,6 +62,14 @@ fvx_status fvx_mesh_line_order(const fvx_mesh_desc* desc, double threshold, int 
 	});
 }
 
+fvx_status fvx_mesh_hilbert(const fvx_mesh_desc* desc, int* perm)
+{
+	return guarded([&] {
+		require(desc && perm, "null argument");
+		fvx::hilbert_order(*desc, perm);
+	});
+}
+
 fvx_status fvx_mesh_create(const fvx_mesh_desc* desc, fvx_mesh* out)
 {
 	return guarded([&] {

@@ -207,6 +207,7 @@ void rcm_order(const fvx_mesh_desc& d, int* perm);
 void find_lines(const fvx_mesh_desc& d, double threshold, std::vector<std::vector<int>>& lines,
                 std::vector<int>& celline);
 void line_order(const fvx_mesh_desc& d, double threshold, int graph_ordering, int* perm);
+void hilbert_order(const fvx_mesh_desc& d, int* perm);
 void build_disc(fvx_disc_s& d);
 GasParams make_gas(const fvx_physics_cfg& p);
 

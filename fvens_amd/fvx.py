@@ -154,6 +154,14 @@ def rcm_permutation(raw: RawMesh) -> np.ndarray:
     return perm
 
 
+def hilbert_permutation(raw: RawMesh) -> np.ndarray:
+    """Cells along a Hilbert curve through their centres (perm[new] = old), computed by libfvx."""
+    d, keep = _desc(raw)
+    perm = np.zeros(raw.nelem, np.int32)
+    check(lib().fvx_mesh_hilbert(C.byref(d), _ptr(perm, C.c_int)))
+    return perm
+
+
 def find_lines(raw: RawMesh, threshold: float):
     """Lines of strongly coupled cells (findLines): a list of cell-index arrays."""
     d, keep = _desc(raw)

@@ -160,6 +160,13 @@ struct MeshData {
 	{
 		if(ordering.empty() || ordering == "natural") return;
 		if(ordering == "rcm") { reorder_rcm(); return; }
+		if(ordering == "hilbert") {          // no reference counterpart: compact patches
+			std::vector<int> perm(nelem());
+			const fvx_mesh_desc d = desc();
+			check(fvx_mesh_hilbert(&d, perm.data()));
+			permute_cells(perm);
+			return;
+		}
 		if(ordering.find("line") != std::string::npos) {
 			if(!(anisotropy_threshold > 0))
 				throw std::invalid_argument("No anisotropy threshold given!");

@@ -98,6 +98,11 @@ fvx_status fvx_mesh_lines(const fvx_mesh_desc* desc, double threshold, int* nlin
  * line-point graph of :264-385 ordered by reverse Cuthill-McKee as PETSc's MatGetOrdering
  * "rcm" does). */
 enum { FVX_LINES_ONLY = 0, FVX_LINES_NATURAL = 1, FVX_LINES_RCM = 2 };
+
+/* Cells sorted along a Hilbert curve through their centres, perm[new] = old. No counterpart in
+ * the reference (its orderings are RCM and the line orderings): a compact-patch order for
+ * callers that want blocks of consecutive cells to share most of their faces. */
+fvx_status fvx_mesh_hilbert(const fvx_mesh_desc* desc, int* perm);
 fvx_status fvx_mesh_line_order(const fvx_mesh_desc* desc, double threshold, int graph_ordering,
                                int* perm);
 

@@ -75,3 +75,48 @@ def test_line_ordering_rejects_bad_arguments(aniso):
     assert rc != 0
     with pytest.raises(KeyError):
         fvx.line_permutation(aniso, 10.0, "line_amd")
+
+
+# ------------------------------------------------------------------ Hilbert order -----------
+
+def _grid(n):
+    """n x n unit squares, cells numbered row by row (a scrambled start for the ordering)."""
+    x = np.arange(n + 1, dtype=float)
+    coords = np.array([[xi, yi] for yi in x for xi in x])
+    nid = lambda i, j: j * (n + 1) + i  # noqa: E731
+    quads = np.array([[nid(i, j), nid(i + 1, j), nid(i + 1, j + 1), nid(i, j + 1)]
+                      for j in range(n) for i in range(n)], np.int32)
+    bface = []
+    for i in range(n):
+        bface += [[nid(i, 0), nid(i + 1, 0)], [nid(i + 1, n), nid(i, n)],
+                  [nid(0, i + 1), nid(0, i)], [nid(n, i), nid(n, i + 1)]]
+    return meshgen.RawMesh(coords=coords, inpoel=quads, nnode=np.full(n * n, 4, np.int32),
+                           bface=np.array(bface, np.int32),
+                           btag=np.full(len(bface), 2, np.int32))
+
+
+@pytest.mark.parametrize("n", [4, 16])
+def test_hilbert_order_walks_a_grid_one_neighbour_at_a_time(n):
+    # on a 2^k x 2^k grid the Hilbert curve visits every cell once and each step moves to an
+    # edge neighbour; it starts in a corner cell
+    raw = _grid(n)
+    perm = fvx.hilbert_permutation(raw)
+    assert sorted(perm) == list(range(n * n))
+    ij = np.stack([perm % n, perm // n], 1)
+    steps = np.abs(np.diff(ij, axis=0)).sum(1)
+    assert np.all(steps == 1)
+    assert tuple(ij[0]) in {(0, 0), (0, n - 1), (n - 1, 0), (n - 1, n - 1)}
+
+
+def test_hilbert_order_makes_compact_blocks():
+    # on the 98,304-cell airfoil O-mesh most interior faces join two cells of the same 64-cell
+    # block of consecutive cells (the Hilbert curve's compact patches: 76%), against 14% in RCM
+    # order, whose blocks are thin strips across the rings
+    import pyoracle as po
+    raw = meshgen.mesh_k2()
+    frac = {}
+    for name, perm in (("hilbert", fvx.hilbert_permutation(raw)), ("rcm", fvx.rcm_permutation(raw))):
+        m = po.Mesh(raw.permuted(perm))
+        f = m.intfac[raw.nbface:, :2]
+        frac[name] = np.mean(f[:, 0] // 64 == f[:, 1] // 64)
+    assert frac["hilbert"] >= 0.7 and frac["rcm"] < 0.3, frac
