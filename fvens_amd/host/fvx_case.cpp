@@ -16,7 +16,7 @@
  *                      [--pc sgs] [--sweeps 16] [--precision f64|f32] [--matrix_free]
  *                      [--matrix_free_difference_step 1e-7]
  *                      [--rcm] [--ksp_max_it 30] [--ksp_rtol 0.1] [--limiter_param 0]
- *                      [--mesh_reorder rcm|line|line_natural|line_rcm]
+ *                      [--mesh_reorder rcm|line|line_natural|line_rcm|hilbert]
  *                      [--mesh_anisotropy_threshold T]
  *
  * The preconditioner, its sweeps and storage precision, matrix-free operation and the cell
