@@ -346,3 +346,36 @@ def test_weno_reconstruction_restated_on_interior_cells():
     for bad in (_weno_cell_residuals(m, d, cfg, u, cells, lam, gam=2.0),
                 _weno_cell_residuals(m, d, cfg, u, cells, 1.0)):
         assert np.abs(r[cells] - bad).max() > 1e3 * 1e-12 * s[cells].max()
+
+
+@pytest.mark.parametrize("flux", ["ROE", "HLLC", "HLL", "AUSM", "AUSMPLUS", "VANLEER"])
+def test_supersonic_upwinding(flux):
+    # upwind fluxes in supersonic flow: with the normal Mach number above 1 on both sides the
+    # numerical flux is the analytic flux of the left state, below -1 that of the right state
+    # (Roe by its A(uR - uL) = F(uR) - F(uL) property with all eigenvalues of one sign; HLL /
+    # HLLC through S_L > 0 or S_R < 0; AUSM, AUSM+ and Van Leer through their split Mach and
+    # pressure functions, which are one-sided beyond |M| = 1). LLF keeps its dissipation and is
+    # not upwind
+    rng = np.random.default_rng(23)
+    cfg = po.make_config(Minf=0.5)
+    g = 1.4
+    for _ in range(20):
+        th = 2 * np.pi * rng.random()
+        n = np.array([np.cos(th), np.sin(th)])
+        t = np.array([-n[1], n[0]])
+        sign = 1.0 if rng.random() < 0.5 else -1.0
+        states = []
+        for _ in range(2):
+            rho = 0.7 + 0.6 * rng.random()
+            p = (0.7 + 0.6 * rng.random()) / (g * 0.25)
+            c = np.sqrt(g * p / rho)
+            vn = sign * (1.3 + 1.2 * rng.random()) * c
+            v = vn * n + 0.2 * c * (rng.random() - 0.5) * t
+            states.append(np.array([rho, rho * v[0], rho * v[1],
+                                    p / (g - 1) + 0.5 * rho * v @ v]))
+        ul, ur = states
+        f = po.flux(cfg, flux, ul, ur, n)
+        want = _analytic(ul if sign > 0 else ur, n)
+        assert np.allclose(f, want, rtol=1e-12, atol=1e-12 * np.abs(want).max()), (flux, f, want)
+    # a planted mistake: the flux of the other side is far from it
+    assert not np.allclose(f, _analytic(ur if sign > 0 else ul, n), rtol=1e-3)
