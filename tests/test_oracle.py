@@ -379,3 +379,24 @@ def test_supersonic_upwinding(flux):
         assert np.allclose(f, want, rtol=1e-12, atol=1e-12 * np.abs(want).max()), (flux, f, want)
     # a planted mistake: the flux of the other side is far from it
     assert not np.allclose(f, _analytic(ur if sign > 0 else ul, n), rtol=1e-3)
+
+
+@pytest.mark.parametrize("flux", ["HLLC", "AUSM", "AUSMPLUS"])
+def test_stationary_contact_is_exact(flux):
+    # a stationary contact (zero normal velocity, equal pressure and tangential velocity,
+    # different densities) is resolved exactly by HLLC (Toro, Riemann Solvers, ch. 10) and by the
+    # AUSM family (zero interface Mach number, split pressures summing to p): the flux is
+    # (0, p n, 0). HLL, LLF and Van Leer smear it, Roe only through its entropy fix at vn = 0
+    rng = np.random.default_rng(5)
+    cfg = po.make_config(Minf=0.5)
+    g = 1.4
+    for _ in range(20):
+        th = 2 * np.pi * rng.random()
+        n = np.array([np.cos(th), np.sin(th)])
+        t = np.array([-n[1], n[0]])
+        p = (0.7 + 0.6 * rng.random()) / (g * 0.25)
+        v = 0.3 * (rng.random() - 0.5) * t
+        ul, ur = (np.array([rho, rho * v[0], rho * v[1], p / (g - 1) + 0.5 * rho * v @ v])
+                  for rho in (0.5 + rng.random(), 0.5 + rng.random()))
+        f = po.flux(cfg, flux, ul, ur, n)
+        assert np.allclose(f, [0.0, p * n[0], p * n[1], 0.0], rtol=0, atol=1e-13 * p), (flux, f)
