@@ -400,3 +400,28 @@ def test_stationary_contact_is_exact(flux):
                   for rho in (0.5 + rng.random(), 0.5 + rng.random()))
         f = po.flux(cfg, flux, ul, ur, n)
         assert np.allclose(f, [0.0, p * n[0], p * n[1], 0.0], rtol=0, atol=1e-13 * p), (flux, f)
+
+
+def test_uniform_grid_viscous_timestep_closed_form():
+    # compute_max_timestep with the viscous term (flow_spatial.cpp:565-634) on uniform h x h
+    # cells in a free stream with constant viscosity 1/Re: every face adds
+    # (|vn| + c) h + max(4/(3 rho), gamma/rho) (1/Re) / Pr h^2 / h^2, so
+    # dt = h^2 / (2 (|vx| + c) h + 2 (|vy| + c) h + 4 max(4/3, gamma) / (Re Pr))
+    nx = ny = 8
+    raw = meshgen.gaussian_bump(nx, ny, 0)
+    x = np.linspace(0, 1, nx + 1); y = np.linspace(0, 1, ny + 1)
+    raw.coords[:] = np.stack(np.meshgrid(x, y, indexing="ij"), axis=-1).reshape(-1, 2)
+    m = po.Mesh(raw)
+    Re, Pr = 200.0, 0.72
+    cfg = po.make_config(Minf=0.5, aoa_deg=30.0, flux="ROE", gradient="NONE", viscous=True,
+                         constvisc=True, Reinf=Re, Pr=Pr, Tinf=288.15,
+                         bcs=[(2, "farfield"), (3, "farfield"), (4, "farfield")])
+    d = po.Disc(m, cfg)
+    u = np.tile(d.freestream(), (m.nelem, 1))
+    _, dt = d.residual(u)
+    h = 1.0 / nx
+    c = 1.0 / 0.5
+    vx, vy = np.cos(np.pi / 6), np.sin(np.pi / 6)
+    visc = 4 * max(4.0 / 3.0, 1.4) / (Re * Pr)
+    expect = h * h / (2 * (abs(vx) + c) * h + 2 * (abs(vy) + c) * h + visc)
+    assert np.allclose(dt, expect, rtol=1e-13)
