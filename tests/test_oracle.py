@@ -425,3 +425,38 @@ def test_uniform_grid_viscous_timestep_closed_form():
     visc = 4 * max(4.0 / 3.0, 1.4) / (Re * Pr)
     expect = h * h / (2 * (abs(vx) + c) * h + 2 * (abs(vy) + c) * h + visc)
     assert np.allclose(dt, expect, rtol=1e-13)
+
+
+def test_subsonic_inflow_ghost_state_relations():
+    # InFlow::computeGhostState (src/spatial/abc.cpp:151-176) on the subsonic inflow marker of
+    # testperiodic.msh with total pressure 5.0 and total temperature 1.04 (non-dimensional): the
+    # ghost state (a) moves along the face normal, (b) is isentropic with those totals,
+    # p_g = p0 (T_g / T0)^(gamma/(gamma-1)) with T_g = gamma M^2 p_g / rho_g, and (c) carries the
+    # interior's outgoing invariant, |v_g| - 2 c_g / (gamma - 1) = R- with c_g^2 = c0^2 T_g / T0,
+    # c0^2 = c_i^2 + (gamma-1)/2 |v_i|^2 and R- = v_i.n - c_i / (2 gamma - 1) as the reference
+    # forms it
+    raw = gmsh.read_gmsh2(golden("testperiodic.msh"))
+    m = po.Mesh(raw)
+    M, g, p0, T0 = 0.5, 1.4, 5.0, 1.04
+    cfg = po.make_config(Minf=M, aoa_deg=3.0, bcs=[(2, "slipwall"), (3, "extrapolation"),
+                                                    (4, "subsonicinflow", (p0, T0))])
+    d = po.Disc(m, cfg)
+    u = states.perturbed_freestream(raw, Minf=M, aoa_deg=3.0)
+    faces = np.flatnonzero(raw.btag == 4)
+    assert len(faces) > 0
+    for b in faces:
+        ui = u[m.intfac[b, 0]]
+        n = m.facemetric[b, :2]
+        ug = d.ghost(b, ui)
+        vg = ug[1:3] / ug[0]
+        vmag = np.linalg.norm(vg)
+        assert abs(vg[0] * n[1] - vg[1] * n[0]) <= 1e-14 * vmag and vg @ n >= 0
+        pg = (g - 1) * (ug[3] - 0.5 * ug[0] * vmag ** 2)
+        Tg = g * M * M * pg / ug[0]
+        assert abs(pg - p0 * (Tg / T0) ** (g / (g - 1))) <= 1e-13 * pg
+        pi = (g - 1) * (ui[3] - 0.5 * (ui[1] ** 2 + ui[2] ** 2) / ui[0])
+        ci = np.sqrt(g * pi / ui[0])
+        rminus = (ui[1:3] @ n) / ui[0] - ci / (2 * g - 1)
+        c02 = ci ** 2 + (g - 1) / 2 * (ui[1] ** 2 + ui[2] ** 2) / ui[0] ** 2
+        cg = np.sqrt(c02 * Tg / T0)
+        assert abs(vmag - 2 * cg / (g - 1) - rminus) <= 1e-12 * abs(rminus)
