@@ -7,9 +7,11 @@ FlowFV::compute_residual (gradients + limiter pass, face-flux / gather pass with
 steps) over a synthetic, seeded perturbed free-stream state resident in HBM.
 
 Prints one JSON line. It also times pseudo-time solves to a residual tolerance (the second
-half of the metric, under "time_to_tol"). The CPU baseline is this repository's CPU oracle in its
-reference-structured OpenMP mode (face loops with atomic scatters), timed on a bounded sample
-of the same workload on the host cores.
+half of the metric, under "time_to_tol": the bump, the airfoil, the reference's laminar case and
+the warm-start A/B), the 64-point batched operating-point sweep ("batch", unless --no-batch) and
+the warm-start network's training and inference kernels ("mpnn"). The CPU baseline is this
+repository's CPU oracle in its reference-structured OpenMP mode (face loops with atomic
+scatters), timed on a bounded sample of the same workload on the host cores.
 """
 from __future__ import annotations
 
