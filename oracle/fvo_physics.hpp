@@ -16,9 +16,11 @@
  *  - HLLC flux               src/spatial/anumericalflux.cpp:1069-1081,1175-1228
  * The Roe and HLLC reference Jacobians (anumericalflux.cpp:736-965, 1230-1397) are the exact
  * derivatives of their fluxes; the oracle obtains them by forward differentiation of the
- * restated flux. Parity: pinned by central finite differences in tests/test_oracle_flux.py,
- * by flux consistency F(u,u,n) = analytic flux, and by the reference wall-flux test
- * (tests/flow-general/testwallbcs.cpp:14-78).
+ * restated flux. Parity: pinned by central finite differences in tests/test_oracle.py, by flux
+ * consistency F(u,u,n) = analytic flux, rotation, supersonic one-sidedness, exact stationary
+ * contacts (HLLC, AUSM, AUSM+), the published subsonic split functions of Van Leer (1982) and
+ * AUSM+ (Liou 1996: beta = 1/8, alpha = 3/16, C1 joins at M = +-1), and by the reference
+ * wall-flux test (tests/flow-general/testwallbcs.cpp:14-78).
  */
 #ifndef FVO_PHYSICS_HPP
 #define FVO_PHYSICS_HPP

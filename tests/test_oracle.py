@@ -5,7 +5,8 @@ Reference tests restated here (paths relative to the FVENS source tree):
   - LeastSquaresGradients_OneExact{,_2,_Unsquad}  tests/finite-volume/testgradientschemes.cpp:36-90
   - SpatialFlow_Walltest_{HLLC,Roe,AUSM,AUSMPlus,HLL,LLF}  tests/flow-general/testwallbcs.cpp:14-78
 Closed forms: flux consistency F(u,u,n) = analytic flux, rotational invariance, freestream
-preservation, conservation, finite-difference checks of every Jacobian.
+preservation, conservation, finite-difference checks of every Jacobian, supersonic upwinding,
+stationary contacts, the published subsonic split functions of Van Leer and AUSM+.
 """
 import numpy as np
 import pytest
@@ -460,3 +461,130 @@ def test_subsonic_inflow_ghost_state_relations():
         c02 = ci ** 2 + (g - 1) / 2 * (ui[1] ** 2 + ui[2] ** 2) / ui[0] ** 2
         cg = np.sqrt(c02 * Tg / T0)
         assert abs(vmag - 2 * cg / (g - 1) - rminus) <= 1e-12 * abs(rminus)
+
+
+def _cons(rho, v, p, g=1.4):
+    return np.array([rho, rho * v[0], rho * v[1], p / (g - 1) + 0.5 * rho * v @ v])
+
+
+def test_vanleer_subsonic_split_fluxes():
+    # Van Leer's splitting (B. van Leer, "Flux-vector splitting for the Euler equations", 1982):
+    # the forward mass flux is rho c (M + 1)^2 / 4 for |M| <= 1, and every component of F+ joins
+    # the supersonic branches with continuous value and slope: F+ = F at M = 1, F+ = 0 at
+    # M = -1. A far-side state moving supersonically away (M > 1 on the right, < -1 on the left)
+    # contributes nothing, which isolates F+ and F- of one subsonic state; F+ + F- = F then
+    # holds for every subsonic state. A dropped term, a wrong sign or factor in the momentum or
+    # energy split breaks the slope match at M = 1.
+    rng = np.random.default_rng(31)
+    cfg = po.make_config(Minf=0.5)
+    g = 1.4
+    for _ in range(10):
+        th = 2 * np.pi * rng.random()
+        n = np.array([np.cos(th), np.sin(th)])
+        t = np.array([-n[1], n[0]])
+        rho = 0.7 + 0.6 * rng.random()
+        p = (0.7 + 0.6 * rng.random()) / (g * 0.25)
+        c = np.sqrt(g * p / rho)
+        vt = 0.4 * c * (rng.random() - 0.5)
+        away_r = _cons(1.0, 3.0 * c * n, p)             # M > 1 on the right: F- = 0
+        away_l = _cons(1.0, -3.0 * c * n, p)            # M < -1 on the left: F+ = 0
+
+        def fplus(M):
+            return po.flux(cfg, "VANLEER", _cons(rho, M * c * n + vt * t, p), away_r, n)
+
+        def fminus(M):
+            return po.flux(cfg, "VANLEER", away_l, _cons(rho, M * c * n + vt * t, p), n)
+
+        scale = rho * c * c * c
+        for M in np.linspace(-0.95, 0.95, 9):
+            u = _cons(rho, M * c * n + vt * t, p)
+            fp = fplus(M)
+            assert abs(fp[0] - rho * c * (M + 1) ** 2 / 4) < 1e-13 * rho * c
+            assert np.allclose(fp + fminus(M), _analytic(u, n), rtol=0, atol=1e-12 * scale)
+        # value and slope at the sonic points, against the supersonic branch just outside
+        h = 1e-5
+        full = lambda M: _analytic(_cons(rho, M * c * n + vt * t, p), n)
+        zero = lambda M: np.zeros(4)
+        for split, edge, outside in ((fplus, 1.0, full), (fplus, -1.0, zero),
+                                     (fminus, -1.0, full), (fminus, 1.0, zero)):
+            inner, outer = edge - np.sign(edge) * h, edge + np.sign(edge) * h
+            assert np.allclose(split(edge), outside(edge), rtol=0, atol=1e-12 * scale)
+            slope_in = (split(edge) - split(inner)) / (edge - inner)
+            slope_out = (outside(outer) - outside(edge)) / (outer - edge)
+            assert np.allclose(slope_in, slope_out, rtol=0, atol=1e-3 * scale), \
+                (split.__name__, edge, slope_in, slope_out)
+    # a planted mistake, on the CPU: the momentum split with +v_n in place of -v_n misses the
+    # full flux at M = 1 by rho c^2 (2/g - 1) n
+    u1 = _cons(rho, c * n + vt * t, p)
+    wrong = rho * c * (u1[1:3] / u1[0] + n * (2 * c + c) / g)
+    assert np.linalg.norm(wrong - full(1.0)[1:3]) > 0.1 * rho * c * c
+
+
+def _ausmplus_split(cfg, rho, M, vt, p, n, side, g=1.4):
+    """Extracts AUSM+'s split Mach and pressure functions of one subsonic state: the far state
+    moves away supersonically and is hot, so the interface speed of sound is the subsonic
+    state's critical speed c* = sqrt(2 (g-1)/(g+1) H) and the far side adds no mass or
+    pressure. Returns (M_split, P_split) at the interface Mach number M = v_n / c*."""
+    t = np.array([-n[1], n[0]])
+    c = np.sqrt(g * p / rho)
+    # v_n = M c*(v_n): c*^2 = 2 (g-1)/(g+1) (c^2/(g-1) + (v_n^2 + v_t^2)/2)
+    k = 2 * (g - 1) / (g + 1)
+    vn = np.sign(M) * np.sqrt(M * M * k * (c * c / (g - 1) + vt * vt / 2) / (1 - M * M * k / 2))
+    u = _cons(rho, vn * n + vt * t, p)
+    H = (u[3] + p) / rho
+    cstar = np.sqrt(k * H)
+    assert abs(vn / cstar - M) < 1e-13
+    far = _cons(0.5, side * 6.0 * c * n, 40.0 * p)      # hot, and supersonic away from the face
+    cfar = np.sqrt(k * (far[3] + 40.0 * p) / 0.5)
+    assert cfar > cstar and 6.0 * c > cstar
+    f = po.flux(cfg, "AUSMPLUS", u, far, n) if side > 0 else po.flux(cfg, "AUSMPLUS", far, u, n)
+    msplit = f[0] / (rho * cstar)
+    psplit = (f[1] * n[0] + f[2] * n[1] - f[0] * vn) / p
+    return msplit, psplit
+
+
+def test_ausmplus_subsonic_split_polynomials():
+    # AUSM+ (M.-S. Liou, "A sequel to AUSM: AUSM+", J. Comput. Phys. 129, 1996): for |M| <= 1
+    # the split Mach number is a quartic and the split pressure a quintic in the interface Mach
+    # number, joined to the supersonic branches (M+ = M, P+ = 1 beyond M = 1; 0 below -1) with
+    # continuous value and slope, and with the published parameters beta = 1/8 (M+(0) = 3/8)
+    # and alpha = 3/16 (P+'(0) = 3/4 + alpha = 15/16); M-(M) = -M+(-M), P-(M) = P+(-M). These
+    # conditions fix both polynomials: the fitted coefficients are checked against them
+    rng = np.random.default_rng(37)
+    cfg = po.make_config(Minf=0.5)
+    g = 1.4
+    Ms = np.linspace(-0.9, 0.9, 13)
+    for _ in range(4):
+        th = 2 * np.pi * rng.random()
+        n = np.array([np.cos(th), np.sin(th)])
+        rho = 0.7 + 0.6 * rng.random()
+        p = (0.7 + 0.6 * rng.random()) / (g * 0.25)
+        vt = 0.3 * np.sqrt(g * p / rho) * (rng.random() - 0.5)
+        plus = np.array([_ausmplus_split(cfg, rho, M, vt, p, n, +1) for M in Ms])
+        minus = np.array([_ausmplus_split(cfg, rho, M, vt, p, n, -1) for M in Ms])
+        mp = np.polynomial.Polynomial.fit(Ms, plus[:, 0], 4).convert()
+        pp = np.polynomial.Polynomial.fit(Ms, plus[:, 1], 5).convert()
+        assert np.abs(mp(Ms) - plus[:, 0]).max() < 1e-12      # a quartic
+        assert np.abs(pp(Ms) - plus[:, 1]).max() < 1e-12      # a quintic
+        assert _liou_conditions_error(mp, pp) < 1e-10
+        # the right-state branch is the mirror image
+        assert np.abs(minus[:, 0] + mp(-Ms)).max() < 1e-12
+        assert np.abs(minus[:, 1] - pp(-Ms)).max() < 1e-12
+    # planted mistakes, on the CPU: AUSM's own split functions (beta = 0 and alpha = 0) meet
+    # every joining condition but fail the published parameters
+    P = np.polynomial.Polynomial
+    ausm_m = P([1, 2, 1]) / 4
+    ausm_p = P([1, 2, 1]) * P([2, -1]) / 4
+    liou_m = ausm_m + P([1, 0, -1]) ** 2 / 8
+    liou_p = ausm_p + 3 / 16 * P([0, 1]) * P([-1, 0, 1]) ** 2
+    assert _liou_conditions_error(liou_m, liou_p) < 1e-14
+    assert _liou_conditions_error(ausm_m, liou_p) > 0.1
+    assert _liou_conditions_error(liou_m, ausm_p) > 0.1
+
+
+def _liou_conditions_error(mp, pp):
+    """Largest violation of the conditions that fix AUSM+'s subsonic split functions."""
+    dm, dp = mp.deriv(), pp.deriv()
+    return max(abs(val - want) for val, want in (
+        (mp(1), 1), (dm(1), 1), (mp(-1), 0), (dm(-1), 0), (mp(0), 3 / 8),
+        (pp(1), 1), (dp(1), 0), (pp(-1), 0), (dp(-1), 0), (pp(0), 0.5), (dp(0), 15 / 16)))
