@@ -27,8 +27,9 @@
  * 1-exactness test (tests/finite-volume/testgradientschemes.cpp:36-90), the wall-flux test
  * (tests/flow-general/testwallbcs.cpp:14-78) and the NACA0012 regression functionals
  * (testcases/naca0012/regr-MUSCL_LeastSquares_HLLC.txt). Jacobian blocks pinned by central
- * finite differences of the restated fluxes and BCs. Time steps: closed form on a uniform
- * Cartesian grid only (parity unpinned beyond that; no reference fixture exists).
+ * finite differences of the restated fluxes and BCs. Time steps: closed forms on a uniform
+ * Cartesian grid (inviscid and viscous) and, with cell-constant face values and a random state
+ * per cell, on the reference's hybrid triangle/quad cylinder mesh.
  */
 #include "fvo.hpp"
 #include "fvo_physics.hpp"

@@ -588,3 +588,44 @@ def _liou_conditions_error(mp, pp):
     return max(abs(val - want) for val, want in (
         (mp(1), 1), (dm(1), 1), (mp(-1), 0), (dm(-1), 0), (mp(0), 3 / 8),
         (pp(1), 1), (dp(1), 0), (pp(-1), 0), (dp(-1), 0), (pp(0), 0.5), (dp(0), 15 / 16)))
+
+
+@pytest.mark.parametrize("viscous", [False, True])
+def test_first_order_timestep_closed_form_on_unstructured_cells(viscous):
+    # compute_max_timestep (flow_spatial.cpp:565-634) with first-order (cell-constant) face
+    # values on the reference's hybrid triangle/quad cylinder mesh and a random state per cell:
+    # each face adds the spectral radius of its own cell's state to that cell, so
+    # dt_c = A_c / sum_e [(|v_c . n_e| + c_c) L_e (+ max(4/(3 rho_c), gamma/rho_c) mu / Pr L_e^2 / A_c)]
+    # over the cell's edges, with A_c and L_e computed here from the node coordinates. A
+    # face's contribution credited to the wrong cell, a dropped |v.n| or a wrong area fails.
+    raw = gmsh.read_gmsh2(golden("2dcylinderhybrid.msh"))
+    m = po.Mesh(raw)
+    g, Re, Pr = 1.4, 150.0, 0.72
+    extra = dict(viscous=True, constvisc=True, Reinf=Re, Pr=Pr, Tinf=288.15) if viscous else {}
+    cfg = po.make_config(Minf=0.5, aoa_deg=10.0, flux="ROE", gradient="NONE",
+                         bcs=[(2, "slipwall"), (4, "farfield")], **extra)
+    d = po.Disc(m, cfg)
+    rng = np.random.default_rng(41)
+    ne = m.nelem
+    rho = 0.7 + 0.6 * rng.random(ne)
+    v = 0.8 * (rng.random((ne, 2)) - 0.5)
+    p = (0.7 + 0.6 * rng.random(ne)) / (g * 0.25)
+    u = np.stack([rho, rho * v[:, 0], rho * v[:, 1],
+                  p / (g - 1) + 0.5 * rho * np.sum(v * v, 1)], 1)
+    _, dt = d.residual(u)
+    c = np.sqrt(g * p / rho)
+    expect = np.empty(ne)
+    for e in range(ne):
+        nodes = raw.coords[raw.inpoel[e, :int(raw.nnode[e])]]
+        nxt = np.roll(nodes, -1, axis=0)
+        area = 0.5 * abs(np.sum(nodes[:, 0] * nxt[:, 1] - nxt[:, 0] * nodes[:, 1]))
+        edge = nxt - nodes
+        L = np.hypot(edge[:, 0], edge[:, 1])
+        normal = np.stack([edge[:, 1], -edge[:, 0]], 1) / L[:, None]
+        s = np.sum((np.abs(normal @ v[e]) + c[e]) * L)
+        if viscous:
+            s += np.sum(max(4 / (3 * rho[e]), g / rho[e]) / Re / Pr * L * L / area)
+        expect[e] = area / s
+    assert np.allclose(dt, expect, rtol=1e-12, atol=0)
+    # a planted mistake, on the CPU: the neighbour's state in place of the cell's own
+    assert not np.allclose(dt, expect * (c / np.roll(c, 1)), rtol=1e-3)
