@@ -6,7 +6,8 @@ Reference tests restated here (paths relative to the FVENS source tree):
   - SpatialFlow_Walltest_{HLLC,Roe,AUSM,AUSMPlus,HLL,LLF}  tests/flow-general/testwallbcs.cpp:14-78
 Closed forms: flux consistency F(u,u,n) = analytic flux, rotational invariance, freestream
 preservation, conservation, finite-difference checks of every Jacobian, supersonic upwinding,
-stationary contacts, the published subsonic split functions of Van Leer and AUSM+.
+stationary contacts, the published subsonic split functions of Van Leer and AUSM+, local time
+steps on uniform and unstructured cells, viscous dissipation of a linear velocity field.
 """
 import numpy as np
 import pytest
@@ -629,3 +630,60 @@ def test_first_order_timestep_closed_form_on_unstructured_cells(viscous):
     assert np.allclose(dt, expect, rtol=1e-12, atol=0)
     # a planted mistake, on the CPU: the neighbour's state in place of the cell's own
     assert not np.allclose(dt, expect * (c / np.roll(c, 1)), rtol=1e-3)
+
+
+def test_viscous_dissipation_of_a_linear_velocity_field():
+    # A linear velocity field v = G x with uniform density and a linear pressure, constant
+    # viscosity mu = 1/Re, on the reference's hybrid cylinder mesh: the least-squares gradients
+    # and the modified average face gradient (aspatial.cpp:172-205) are exact, so the stress and
+    # the heat flux are uniform. Summed over a closed cell, the viscous momentum flux then
+    # vanishes and so does the heat flux, while the work of the stress integrates exactly
+    # (midpoint rule on straight faces, v linear) to the viscous dissipation:
+    #   A mu [2 (u_x^2 + v_y^2) + (u_y + v_x)^2 - 2/3 (u_x + v_y)^2]
+    # in the energy equation (textbook dissipation function, Stokes' hypothesis). The viscous
+    # part is the difference of the viscous and inviscid residuals; cells three layers from the
+    # boundary are checked (ghost states make the outer layers' gradients inexact). A missing
+    # transpose in the stress, a wrong 2/3 term or a sign error fails.
+    raw = gmsh.read_gmsh2(golden("2dcylinderhybrid.msh"))
+    m = po.Mesh(raw)
+    g, Re, Pr = 1.4, 50.0, 0.72
+    bcs = [(2, "slipwall"), (4, "farfield")]
+    common = dict(Minf=0.5, aoa_deg=0.0, flux="ROE", gradient="LEASTSQUARES", recon="NONE",
+                  bcs=bcs)
+    dv = po.Disc(m, po.make_config(viscous=True, constvisc=True, Reinf=Re, Pr=Pr, Tinf=288.15,
+                                   **common))
+    di = po.Disc(m, po.make_config(**common))
+    rng = np.random.default_rng(43)
+    ne = m.nelem
+    G = 0.3 * (rng.random((2, 2)) - 0.5)
+    x = m.rc
+    v = 0.2 + x @ G.T
+    rho = np.full(ne, 1.1)
+    p = 1.0 / (g * 0.25) + x @ (0.05 * (rng.random(2) - 0.5))
+    u = np.stack([rho, rho * v[:, 0], rho * v[:, 1],
+                  p / (g - 1) + 0.5 * rho * np.sum(v * v, 1)], 1)
+    visc = dv.residual(u)[0] - di.residual(u)[0]
+    # cells at least three neighbour layers away from any ghost
+    near = np.zeros(ne, bool)
+    for e in range(ne):
+        near[e] = np.any(m.esuel[e, :int(raw.nnode[e])] >= ne)
+    for _ in range(3):
+        grow = near.copy()
+        for e in range(ne):
+            nb = m.esuel[e, :int(raw.nnode[e])]
+            nb = nb[nb < ne]
+            if np.any(near[nb]):
+                grow[e] = True
+        near = grow
+    cells = np.nonzero(~near)[0]
+    assert len(cells) > 50
+    ux, uy, vx, vy = G[0, 0], G[0, 1], G[1, 0], G[1, 1]
+    phi = (2 * (ux ** 2 + vy ** 2) + (uy + vx) ** 2 - 2 / 3 * (ux + vy) ** 2) / Re
+    scale = np.abs(visc[cells]).max()
+    assert np.abs(visc[cells, :3]).max() < 1e-12 * max(scale, phi * m.area.max())
+    assert np.allclose(visc[cells, 3], m.area[cells] * phi, rtol=1e-9, atol=0), \
+        np.abs(visc[cells, 3] / (m.area[cells] * phi) - 1).max()
+    # a planted mistake, on the CPU: without the transpose in the stress the work is
+    # mu (G : G - 2/3 tr(G)^2), short by mu G^T : G
+    wrong = (np.sum(G * G) - 2 / 3 * (ux + vy) ** 2) / Re
+    assert abs(wrong - phi) > 1e-2 * phi
