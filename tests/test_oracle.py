@@ -7,7 +7,8 @@ Reference tests restated here (paths relative to the FVENS source tree):
 Closed forms: flux consistency F(u,u,n) = analytic flux, rotational invariance, freestream
 preservation, conservation, finite-difference checks of every Jacobian, supersonic upwinding,
 stationary contacts, the published subsonic split functions of Van Leer and AUSM+, local time
-steps on uniform and unstructured cells, viscous dissipation of a linear velocity field.
+steps on uniform and unstructured cells, viscous dissipation of a linear velocity field,
+Barth-Jespersen limiting of a step to first order.
 """
 import numpy as np
 import pytest
@@ -687,3 +688,38 @@ def test_viscous_dissipation_of_a_linear_velocity_field():
     # mu (G : G - 2/3 tr(G)^2), short by mu G^T : G
     wrong = (np.sum(G * G) - 2 / 3 * (ux + vy) ** 2) / Re
     assert abs(wrong - phi) > 1e-2 * phi
+
+
+@pytest.mark.parametrize("gradient", ["LEASTSQUARES", "GREENGAUSS"])
+def test_barth_jespersen_limits_a_step_to_first_order(gradient):
+    # Barth-Jespersen (limitedlinearreconstruction.cpp) scales each slope so that the face
+    # values stay within the min / max of the cell and its neighbours. For a density step
+    # across x = 1/2 on a uniform grid (uniform velocity and pressure, extrapolation boundaries
+    # so ghosts repeat their cells) the cells beside the step hold the extreme value of their
+    # neighbourhood and their slope towards the other side would overshoot, so the limiter
+    # factor is exactly 0 there; every other cell has a zero gradient. The limited
+    # second-order residual therefore equals the first-order residual in every cell, while
+    # the unlimited one does not.
+    nx = ny = 8
+    raw = meshgen.gaussian_bump(nx, ny, 0)
+    x = np.linspace(0, 1, nx + 1); y = np.linspace(0, 1, ny + 1)
+    raw.coords[:] = np.stack(np.meshgrid(x, y, indexing="ij"), axis=-1).reshape(-1, 2)
+    m = po.Mesh(raw)
+    bcs = [(2, "extrapolation"), (3, "extrapolation"), (4, "extrapolation")]
+    common = dict(Minf=0.5, aoa_deg=20.0, flux="HLLC", bcs=bcs)
+    res = {}
+    for name, kw in (("bj", dict(gradient=gradient, recon="BARTHJESPERSEN")),
+                     ("first", dict(gradient="NONE")),
+                     ("unlimited", dict(gradient=gradient, recon="NONE"))):
+        d = po.Disc(m, po.make_config(**common, **kw))
+        uinf = d.freestream()
+        u = np.tile(uinf, (m.nelem, 1))
+        vinf = uinf[1:3] / uinf[0]
+        pinf = 0.4 * (uinf[3] - 0.5 * uinf[0] * vinf @ vinf)
+        u[m.rc[:, 0] > 0.5] = _cons(1.8 * uinf[0], vinf, pinf)   # density jump, same v and p
+        res[name] = d.residual(u)[0]
+    scale = np.abs(res["first"]).max()
+    assert scale > 1e-3
+    assert np.allclose(res["bj"], res["first"], rtol=0, atol=1e-13 * scale)
+    # the planted mistake: no limiter at all
+    assert not np.allclose(res["unlimited"], res["first"], rtol=0, atol=1e-3 * scale)
