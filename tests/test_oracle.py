@@ -8,7 +8,7 @@ Closed forms: flux consistency F(u,u,n) = analytic flux, rotational invariance, 
 preservation, conservation, finite-difference checks of every Jacobian, supersonic upwinding,
 stationary contacts, the published subsonic split functions of Van Leer and AUSM+, local time
 steps on uniform and unstructured cells, viscous dissipation of a linear velocity field,
-Barth-Jespersen limiting of a step to first order.
+Barth-Jespersen and Venkatakrishnan limiting of a step (first-order, unlimited and K^3 limits).
 """
 import numpy as np
 import pytest
@@ -700,26 +700,54 @@ def test_barth_jespersen_limits_a_step_to_first_order(gradient):
     # factor is exactly 0 there; every other cell has a zero gradient. The limited
     # second-order residual therefore equals the first-order residual in every cell, while
     # the unlimited one does not.
+    step = _step_residual_fn()
+    res = {"bj": step(gradient=gradient, recon="BARTHJESPERSEN"),
+           "first": step(gradient="NONE"),
+           "unlimited": step(gradient=gradient, recon="NONE")}
+    scale = np.abs(res["first"]).max()
+    assert scale > 1e-3
+    assert np.allclose(res["bj"], res["first"], rtol=0, atol=1e-13 * scale)
+    # the planted mistake: no limiter at all
+    assert not np.allclose(res["unlimited"], res["first"], rtol=0, atol=1e-3 * scale)
+
+
+def _step_residual_fn():
+    """Residual of a density step across x = 1/2 (uniform velocity and pressure) on a uniform
+    8 x 8 grid with extrapolation boundaries, as a function of the numerics options."""
     nx = ny = 8
     raw = meshgen.gaussian_bump(nx, ny, 0)
     x = np.linspace(0, 1, nx + 1); y = np.linspace(0, 1, ny + 1)
     raw.coords[:] = np.stack(np.meshgrid(x, y, indexing="ij"), axis=-1).reshape(-1, 2)
     m = po.Mesh(raw)
     bcs = [(2, "extrapolation"), (3, "extrapolation"), (4, "extrapolation")]
-    common = dict(Minf=0.5, aoa_deg=20.0, flux="HLLC", bcs=bcs)
-    res = {}
-    for name, kw in (("bj", dict(gradient=gradient, recon="BARTHJESPERSEN")),
-                     ("first", dict(gradient="NONE")),
-                     ("unlimited", dict(gradient=gradient, recon="NONE"))):
-        d = po.Disc(m, po.make_config(**common, **kw))
+
+    def residual(**kw):
+        d = po.Disc(m, po.make_config(Minf=0.5, aoa_deg=20.0, flux="HLLC", bcs=bcs, **kw))
         uinf = d.freestream()
         u = np.tile(uinf, (m.nelem, 1))
         vinf = uinf[1:3] / uinf[0]
         pinf = 0.4 * (uinf[3] - 0.5 * uinf[0] * vinf @ vinf)
-        u[m.rc[:, 0] > 0.5] = _cons(1.8 * uinf[0], vinf, pinf)   # density jump, same v and p
-        res[name] = d.residual(u)[0]
-    scale = np.abs(res["first"]).max()
-    assert scale > 1e-3
-    assert np.allclose(res["bj"], res["first"], rtol=0, atol=1e-13 * scale)
-    # the planted mistake: no limiter at all
-    assert not np.allclose(res["unlimited"], res["first"], rtol=0, atol=1e-3 * scale)
+        u[m.rc[:, 0] > 0.5] = _cons(1.8 * uinf[0], vinf, pinf)
+        return d.residual(u)[0]
+    return residual
+
+
+def test_venkatakrishnan_limits_on_a_step():
+    # Venkatakrishnan's limiter phi = (dp^2 + 2 dp dm + eps^2) / (dp^2 + dp dm + 2 dm^2 + eps^2)
+    # with eps^2 = (K h)^3 (limitedlinearreconstruction.cpp): at the extremum cells beside a
+    # density step dp = 0, so phi = eps^2 / (2 dm^2 + eps^2). K = 0 gives exactly the
+    # first-order residual, a huge K exactly the unlimited one, and for small K the departure
+    # from first order is linear in phi, i.e. proportional to K^3: doubling K multiplies it by 8
+    # (an eps^2 quadratic in K would give 4)
+    step = _step_residual_fn()
+    first = step(gradient="NONE")
+    unlimited = step(gradient="LEASTSQUARES", recon="NONE")
+    scale = np.abs(first).max()
+    venk = lambda K: step(gradient="LEASTSQUARES", recon="VENKATAKRISHNAN", limiter_param=K)
+    assert np.allclose(venk(0.0), first, rtol=0, atol=1e-13 * scale)
+    assert np.allclose(venk(1e6), unlimited, rtol=0, atol=1e-13 * scale)
+    d1, d2 = venk(0.01) - first, venk(0.02) - first
+    assert np.abs(d1).max() > 1e-10 * scale
+    assert np.allclose(d2, 8 * d1, rtol=1e-4, atol=1e-4 * np.abs(d2).max())
+    # the planted mistake: eps^2 quadratic in K
+    assert not np.allclose(d2, 4 * d1, rtol=1e-2, atol=1e-4 * np.abs(d2).max())
