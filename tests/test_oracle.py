@@ -6,7 +6,7 @@ Reference tests restated here (paths relative to the FVENS source tree):
   - SpatialFlow_Walltest_{HLLC,Roe,AUSM,AUSMPlus,HLL,LLF}  tests/flow-general/testwallbcs.cpp:14-78
 Closed forms: flux consistency F(u,u,n) = analytic flux, rotational invariance, freestream
 preservation, conservation, finite-difference checks of every Jacobian, supersonic upwinding,
-stationary contacts, the published subsonic split functions of Van Leer and AUSM+, local time
+stationary contacts, the published subsonic split functions of Van Leer, AUSM and AUSM+, local time
 steps on uniform and unstructured cells, viscous dissipation of a linear velocity field,
 Barth-Jespersen and Venkatakrishnan limiting of a step (first-order, unlimited and K^3 limits).
 """
@@ -751,3 +751,44 @@ def test_venkatakrishnan_limits_on_a_step():
     assert np.allclose(d2, 8 * d1, rtol=1e-4, atol=1e-4 * np.abs(d2).max())
     # the planted mistake: eps^2 quadratic in K
     assert not np.allclose(d2, 4 * d1, rtol=1e-2, atol=1e-4 * np.abs(d2).max())
+
+
+def test_ausm_subsonic_split_polynomials():
+    # Liou-Steffen AUSM (J. Comput. Phys. 107, 1993) uses van Leer's split Mach number
+    # M+ = (M + 1)^2 / 4 and pressure P+ = (M + 1)^2 (2 - M) / 4 for |M| <= 1, in the state's own
+    # speed of sound. A quadratic meeting the supersonic branches (M+ = M beyond 1, 0 below -1)
+    # with matching value and slope, and a cubic doing the same for P+ (1 and 0), are unique, so
+    # the degrees and the joining conditions pin both. One side is isolated as for AUSM+: the
+    # far state moves away supersonically (its Mach number beyond 1 in its own sound speed)
+    rng = np.random.default_rng(47)
+    cfg = po.make_config(Minf=0.5)
+    g = 1.4
+    Ms = np.linspace(-0.9, 0.9, 11)
+    for _ in range(4):
+        th = 2 * np.pi * rng.random()
+        n = np.array([np.cos(th), np.sin(th)])
+        t = np.array([-n[1], n[0]])
+        rho = 0.7 + 0.6 * rng.random()
+        p = (0.7 + 0.6 * rng.random()) / (g * 0.25)
+        c = np.sqrt(g * p / rho)
+        vt = 0.3 * c * (rng.random() - 0.5)
+        for side in (+1, -1):
+            ms, ps = [], []
+            for M in Ms:
+                u = _cons(rho, M * c * n + vt * t, p)
+                far = _cons(0.9, side * 2.5 * c * n, 0.9 * p)
+                f = po.flux(cfg, "AUSM", u, far, n) if side > 0 else po.flux(cfg, "AUSM", far, u, n)
+                ms.append(f[0] / (rho * c))
+                ps.append((f[1] * n[0] + f[2] * n[1] - f[0] * M * c) / p)
+            ms, ps = np.array(ms), np.array(ps)
+            if side < 0:                       # the right-state branch, mirrored: M- (M) = -M+ (-M)
+                ms, ps, x = -ms, ps, -Ms
+            else:
+                x = Ms
+            mp = np.polynomial.Polynomial.fit(x, ms, 2).convert()
+            pp = np.polynomial.Polynomial.fit(x, ps, 3).convert()
+            assert np.abs(mp(x) - ms).max() < 1e-12 and np.abs(pp(x) - ps).max() < 1e-12
+            dm, dp = mp.deriv(), pp.deriv()
+            for val, want in ((mp(1), 1), (dm(1), 1), (mp(-1), 0), (dm(-1), 0),
+                              (pp(1), 1), (dp(1), 0), (pp(-1), 0), (dp(-1), 0)):
+                assert abs(val - want) < 1e-10, (side, val, want)
