@@ -20,8 +20,7 @@
  * consistency F(u,u,n) = analytic flux, rotation, supersonic one-sidedness, exact stationary
  * contacts (HLLC, AUSM, AUSM+), the published subsonic split functions of Van Leer (1982), AUSM
  * (Liou-Steffen 1993) and AUSM+ (Liou 1996: beta = 1/8, alpha = 3/16), C1 joins at M = +-1,
- * and by the reference
- * wall-flux test (tests/flow-general/testwallbcs.cpp:14-78).
+ * and by the reference wall-flux test (tests/flow-general/testwallbcs.cpp:14-78).
  */
 #ifndef FVO_PHYSICS_HPP
 #define FVO_PHYSICS_HPP
