@@ -18,7 +18,10 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import statistics
+import subprocess
 import sys
+import tempfile
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -39,7 +42,11 @@ def parse():
     p.add_argument("--warmup", type=int, default=20)
     p.add_argument("--cpu-seconds", type=float, default=12.0,
                    help="bounded CPU-baseline sample (seconds of CPU work)")
+    p.add_argument("--cpu-seconds-1t", type=float, default=8.0,
+                   help="bounded one-thread CPU-baseline sample (seconds)")
     p.add_argument("--no-cpu-baseline", action="store_true")
+    p.add_argument("--order", choices=["rcm", "hilbert"], default="rcm",
+                   help="cell ordering of the residual workload")
     p.add_argument("--no-batch", action="store_true",
                    help="skip the batched operating-point sweep")
     p.add_argument("--batch-instances", type=int, default=64)
@@ -75,11 +82,12 @@ def calibrate_copy(steps):
                       "copy_GBs": round(2 * n * 8 / dt / 1e9, 1), "launches": steps + 3}))
 
 
-def build_case():
+def build_case(order="rcm"):
     import numpy as np
     from fvens_amd import fvx, meshgen, states
     raw = meshgen.mesh_k2l()
-    raw = raw.permuted(fvx.rcm_permutation(raw))
+    perm = fvx.rcm_permutation(raw) if order == "rcm" else fvx.hilbert_permutation(raw)
+    raw = raw.permuted(perm)
     bcs = [(2, "slipwall"), (4, "farfield")]
     pcfg = fvx.physics_cfg(Minf=0.8, aoa_deg=1.25, bcs=bcs)
     ncfg = fvx.numerics_cfg(flux="HLLC", gradient="LEASTSQUARES", recon="VENKATAKRISHNAN",
@@ -347,24 +355,69 @@ def mpnn_timing(graphs, steps):
             "train_rows_per_s": round(rows / tr * 1e3), "infer_rows_per_s": round(rows / inf * 1e3)}
 
 
-def cpu_baseline(raw, bcs, u, budget_s):
+def host_cpu():
+    """CPU model, sockets x cores per socket, threads per core, logical CPUs and the CPUs
+    this process may run on."""
+    info = {}
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=20).stdout
+        for line in out.splitlines():
+            k, _, v = line.partition(":")
+            info[k.strip()] = v.strip()
+    except (OSError, subprocess.SubprocessError):
+        pass
+    return {"model": info.get("Model name"), "sockets": info.get("Socket(s)"),
+            "cores_per_socket": info.get("Core(s) per socket"),
+            "threads_per_core": info.get("Thread(s) per core"), "nproc": os.cpu_count(),
+            "affinity": len(os.sched_getaffinity(0))}
+
+
+def native_oracle():
+    """Builds the CPU oracle with -O3 -march=native for this machine (a few seconds) in a
+    temporary directory and loads it; falls back to the portable -O3 build."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po
+    out = tempfile.mkdtemp(prefix="fvo_native_")
+    p = subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "-s", "native",
+                        f"NATIVE_DIR={out}"], capture_output=True, text=True)
+    if p.returncode == 0:
+        po.use_library(os.path.join(out, "liboracle.so"))
+        return po, "g++ -O3 -march=native -fopenmp"
+    return po, "g++ -O3 -fopenmp (portable build; the native build failed)"
+
+
+def cpu_baseline(po, flags, raw, bcs, u, budget_s, budget_1t):
+    """The oracle's reference-structured OpenMP residual (face loops with atomic scatters) on
+    the same operator and state: all the host cores this process may use, then one thread."""
     m = po.Mesh(raw)
     d = po.Disc(m, po.make_config(Minf=0.8, aoa_deg=1.25, flux="HLLC", gradient="LEASTSQUARES",
                                   recon="VENKATAKRISHNAN", limiter_param=5.0, bcs=bcs))
-    d.residual(u, parallel=1)                       # warm-up
-    n, t = 0, 0.0
-    while t < budget_s and n < 50:
-        t0 = time.perf_counter()
-        d.residual(u, parallel=1)
-        t += time.perf_counter() - t0
-        n += 1
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-    return dict(value=raw.nelem * n / t / 1e6, unit="Mcell-residuals/s", cores=threads,
+    threads = len(os.sched_getaffinity(0))
+
+    def run(nthreads, budget, nmax):
+        po.set_threads(nthreads)
+        d.residual(u, parallel=1)                   # warm-up
+        times = []
+        while sum(times) < budget and len(times) < nmax:
+            t0 = time.perf_counter()
+            d.residual(u, parallel=1)
+            times.append(time.perf_counter() - t0)
+        return times
+
+    tall = run(threads, budget_s, 50)
+    t1 = run(1, budget_1t, 10)
+    po.set_threads(threads)
+    med = statistics.median(tall)
+    med1 = statistics.median(t1)
+    return dict(value=raw.nelem / med / 1e6, unit="Mcell-residuals/s", cores=threads,
                 kind="port",
-                sample=f"{n} full residual evaluations of the same 1,572,864-cell operator and "
-                       f"state ({t:.1f} s), oracle OpenMP face loops with atomic scatters")
+                sample=f"median of {len(tall)} full residual evaluations of the same "
+                       f"{raw.nelem:,}-cell operator and state ({sum(tall):.1f} s), oracle "
+                       f"OpenMP face loops with atomic scatters, {threads} threads "
+                       f"(OMP_PLACES/OMP_PROC_BIND as set), {flags}",
+                one_thread={"value": raw.nelem / med1 / 1e6, "evaluations": len(t1),
+                            "seconds": round(sum(t1), 2)},
+                host=host_cpu())
 
 
 def main():
@@ -390,7 +443,7 @@ def main():
         a.no_batch = True
     elif a.only == "residual":
         a.no_batch = a.no_solve = True
-    raw, bcs, pcfg, ncfg, u_host = build_case()
+    raw, bcs, pcfg, ncfg, u_host = build_case(a.order)
     mesh = fvx.Mesh(raw)
     disc = fvx.FlowFV(mesh, pcfg, ncfg)
     u = torch.tensor(u_host, device="cuda")
@@ -398,14 +451,27 @@ def main():
     dt = torch.zeros(raw.nelem, dtype=torch.float64, device="cuda")
     for _ in range(a.warmup):
         disc.compute_residual(u, r, True, dt)
-    torch.cuda.synchronize()
-    disc.profile_enable(True)
+    # the timed region: exactly `steps` residual evaluations, nothing else recorded
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(a.steps):
         disc.compute_residual(u, r, True, dt)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
+    # median over >= 50 evaluations, each bracketed by events on the launch stream
+    nmed = max(50, a.steps)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(nmed)]
+    for e0, e1 in evs:
+        e0.record()
+        disc.compute_residual(u, r, True, dt)
+        e1.record()
+    torch.cuda.synchronize()
+    med_ms = statistics.median(e0.elapsed_time(e1) for e0, e1 in evs)
+    # per-kernel device times from the library's own event pairs, in a loop of their own
+    disc.profile_enable(True)
+    for _ in range(nmed):
+        disc.compute_residual(u, r, True, dt)
     flux_ms, flux_n = disc.profile_read("flux")
     grad_ms, grad_n = disc.profile_read("gradients")
     disc.profile_enable(False)
@@ -419,7 +485,9 @@ def main():
     tf = os.path.join(ROOT, "profiles", "k_flux_traffic.json")
     if os.path.exists(tf):
         with open(tf) as fh:
-            traffic = json.load(fh).get("hbm_bytes_per_launch")
+            tj = json.load(fh)
+        if tj.get("order", "rcm") == a.order and tj.get("cells") == raw.nelem:
+            traffic = tj.get("hbm_bytes_per_launch")
     out = {
         "metric": "Mcell-residuals/s; time to 1e-10 residual: warm-start+FP32 PC vs plain FP64",
         "value": round(value, 3),
@@ -429,13 +497,15 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": "inviscid transonic airfoil residual, NACA 4-digit t=0.12 O-mesh "
-                               "2048x512 (outer 256 rings split), RCM order",
+                               f"2048x512 (outer 256 rings split), {a.order.upper()} order",
                    "cells": raw.nelem, "faces": int(mesh.naface), "flux": "HLLC",
                    "gradient": "weighted least squares", "limiter": "Venkatakrishnan K=5",
                    "mach": 0.8, "aoa_deg": 1.25, "state": "seeded perturbed free stream",
                    "parallelism": "single GPU"},
+        "median_ms_per_evaluation": round(med_ms, 5),
+        "median_value": round(raw.nelem / med_ms / 1e3, 3),
         "kernels_ms": {"k_gradients": round(grad_avg_s * 1e3, 5),
-                       "k_flux": round(flux_avg_s * 1e3, 5)},
+                       "k_flux": round(flux_avg_s * 1e3, 5), "evaluations": nmed},
         "roofline": {"bound": "hbm", "kernel": "k_flux", "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -443,8 +513,11 @@ def main():
         "residual_roofline_frac": round(RESIDUAL_BYTES_PER_CELL * raw.nelem
                                         / ((flux_avg_s + grad_avg_s) * 1e9) / HBM_PEAK_GBS, 4),
     }
+    po = None
     if not a.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(raw, bcs, u_host, a.cpu_seconds)
+        po, flags = native_oracle()
+        out["cpu_baseline"] = cpu_baseline(po, flags, raw, bcs, u_host, a.cpu_seconds,
+                                           a.cpu_seconds_1t)
     if not a.no_solve:
         # second half of the metric: time to 1e-10 of the free-stream residual norm
         del disc, mesh, u, r, dt

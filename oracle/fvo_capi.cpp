@@ -213,3 +213,12 @@ extern "C" int fvo_scalar_face_values(void* dp, const double* u, const double* u
 {
 	FVO_TRY(fvo::scalar_face_values(*static_cast<fvo::Disc*>(dp), u, ug, grads, ufl, ufr))
 }
+
+namespace fvo {
+void reconstructed_face_states(const Disc& d, const double* u, double* ufl, double* ufr);
+}
+
+extern "C" int fvo_face_states(void* dp, const double* u, double* ufl, double* ufr)
+{
+	FVO_TRY(fvo::reconstructed_face_states(*static_cast<fvo::Disc*>(dp), u, ufl, ufr))
+}

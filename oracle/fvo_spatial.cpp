@@ -789,6 +789,27 @@ void compute_primitive_gradients(const Disc& d, const double* u, double* grads, 
 	compute_gradients(d, up.data(), ug.data(), grads, 4, par);
 }
 
+/* Primitive face states of the second-order reconstruction (ufl, ufr [naface][4]; a boundary
+ * face has its left state only), the values residual_impl converts back to conserved before the
+ * flux. Exposed so that tests can read a limiter's factor off a face value. */
+void reconstructed_face_states(const Disc& d, const double* u, double* ufl, double* ufr)
+{
+	const Mesh& m = *d.m;
+	const Gas gs(d.cfg);
+	const int ne = m.nelem, nb = m.nbface;
+	std::vector<double> ug((size_t)nb*4), up((size_t)ne*4), grads((size_t)ne*8, 0.0);
+	for(int f = 0; f < nb; f++) {
+		double g[4];
+		ghost_state(d, f, &u[m.intfac[f*4]*4], &m.facemetric[f*3], g);
+		primitive_from_conserved(gs, g, &ug[f*4]);
+	}
+	for(int c = 0; c < ne; c++) primitive_from_conserved(gs, &u[c*4], &up[c*4]);
+	compute_gradients(d, up.data(), ug.data(), grads.data(), 4, 0);
+	std::fill(ufl, ufl + (size_t)m.naface*4, 0.0);
+	std::fill(ufr, ufr + (size_t)m.naface*4, 0.0);
+	reconstruct(d, up.data(), ug.data(), grads.data(), ufl, ufr, 0);
+}
+
 void compute_conserved_gradients(const Disc& d, const double* u, double* grads)
 {
 	const Mesh& m = *d.m;

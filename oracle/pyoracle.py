@@ -42,10 +42,28 @@ class SolverConfig(C.Structure):
                 ("fixed_norm", C.c_double), ("pc_sweeps", C.c_int)]
 
 
+_PATH = None
+
+
+def use_library(path):
+    """Loads the oracle from `path` (the CPU-baseline build tuned for the timing machine)
+    instead of oracle/liboracle.so; must be called before the first use."""
+    global _PATH
+    if _LIB is not None and path != _PATH:
+        raise RuntimeError("oracle library already loaded")
+    _PATH = path
+
+
+def set_threads(n):
+    """OpenMP thread count of the oracle's parallel loops started from this thread."""
+    lib()
+    C.CDLL("libgomp.so.1").omp_set_num_threads(int(n))
+
+
 def lib():
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+        path = _PATH or os.path.join(_HERE, "liboracle.so")
         if not os.path.exists(path):
             raise RuntimeError(f"oracle library not built: {path} (run __graft_entry__.build())")
         L = C.CDLL(path)
@@ -178,6 +196,15 @@ class Disc:
                                             _p(g, C.c_double), _p(ufl, C.c_double),
                                             _p(ufr, C.c_double)))
         return g, ufl, ufr
+
+    def face_states(self, u):
+        """Primitive face states (left, right) of the second-order reconstruction."""
+        u = np.ascontiguousarray(u, np.float64)
+        ufl = np.zeros((self.mesh.naface, 4))
+        ufr = np.zeros((self.mesh.naface, 4))
+        _check(lib().fvo_face_states(self.h, _p(u, C.c_double), _p(ufl, C.c_double),
+                                     _p(ufr, C.c_double)))
+        return ufl, ufr
 
     def prim_gradients(self, u, parallel=0):
         u = np.ascontiguousarray(u, np.float64)
