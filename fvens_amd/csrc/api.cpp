@@ -1,9 +1,11 @@
 /* api.cpp -- the extern "C" boundary of libfvx (declared in include/fvx.h). Converts internal
  * exceptions to status codes and keeps the last error text per thread. */
 #include "fvx_internal.hpp"
+#include "wave.hpp"
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <functional>
 
 namespace {
 thread_local std::string g_last_error;
@@ -21,6 +23,11 @@ void require(bool c, const char* msg)
 {
 	if(!c) throw fvx::Error(FVX_ERR_INVALID, msg);
 }
+}
+
+namespace fvx {
+/// The status conversion of this boundary for entry points defined in other source files.
+fvx_status guard(const std::function<void()>& f) { return guarded(f); }
 }
 
 extern "C" {
@@ -310,13 +317,14 @@ fvx_status fvx_matfree_apply(fvx_disc d, const double* u, const double* r_u, con
 		require(eps > 0, "difference step must be positive");
 		hipStream_t s = static_cast<hipStream_t>(stream);
 		const size_t N = 4*(size_t)d->m->nelem;
-		fvx::DBuf<double> aux, raux, part;
-		aux.alloc(N);
-		raux.alloc(N);
-		part.alloc((size_t)fvx::partial_stride() + 1);
-		fvx::matfree_apply(*d, u, r_u, dtm, x, y, eps, aux.p, raux.p, part.p,
-		                   part.p + (size_t)fvx::partial_stride(), s);
-		FVX_HIP(hipStreamSynchronize(s));
+		// handle-owned scratch, sized once; later calls reuse it in stream order
+		if(d->mf_aux.n != N) {
+			d->mf_aux.alloc(N);
+			d->mf_raux.alloc(N);
+			d->mf_part.alloc((size_t)fvx::partial_stride() + 1);
+		}
+		fvx::matfree_apply(*d, u, r_u, dtm, x, y, eps, d->mf_aux.p, d->mf_raux.p, d->mf_part.p,
+		                   d->mf_part.p + (size_t)fvx::partial_stride(), s);
 	});
 }
 
@@ -431,23 +439,28 @@ fvx_status fvx_batch_residual_norms(fvx_batch b, const double* u, double* out, v
 		hipStream_t s = static_cast<hipStream_t>(stream);
 		fvx_disc_s& d = *b->d;
 		const int n = d.m->nelem;
-		fvx::DBuf<double> r, part, nrm;
-		r.alloc((size_t)4*n*b->ninst);
-		part.alloc((size_t)fvx::partial_stride()*b->ninst);
-		nrm.alloc(b->ninst);
-		FVX_HIP(hipMemsetAsync(r.p, 0, r.n*sizeof(double), s));
-		fvx::BatchArg ba;
-		ba.list = b->d_all.p;
-		ba.gp = b->d_gas.p;
-		fvx::launch_residual(d, u, r.p, 0, nullptr, s, ba, b->ninst, b->d_grad.p, b->d_rgrad.p);
+		// batch-owned scratch, allocated on the first call
+		if(!b->n_host) {
+			b->n_r.alloc((size_t)4*n*b->ninst);
+			b->n_part.alloc((size_t)fvx::partial_stride()*b->ninst);
+			b->n_out.alloc(b->ninst);
+			FVX_HIP(hipHostMalloc((void**)&b->n_host, b->ninst*sizeof(double)));
+		}
 		fvx::Inst L;
 		L.list = b->d_all.p;
 		L.nact = b->ninst;
-		fvx::launch_resnorm(n, r.p, d.m->d_area.p, part.p, nrm.p, 1, L, s);
-		std::vector<double> h(b->ninst);
-		FVX_HIP(hipMemcpyAsync(h.data(), nrm.p, h.size()*sizeof(double), hipMemcpyDeviceToHost, s));
+		fvx::launch_zero(4L*n, b->n_r.p, 4L*n, L, s);
+		fvx::BatchArg ba;
+		ba.list = b->d_all.p;
+		ba.gp = b->d_gas.p;
+		fvx::launch_residual(d, u, b->n_r.p, 0, nullptr, s, ba, b->ninst, b->d_grad.p,
+		                     b->d_rgrad.p);
+		fvx::launch_resnorm(n, b->n_r.p, d.m->d_area.p, b->n_part.p, b->n_out.p, 1, L, s);
+		FVX_HIP(hipMemcpyAsync(b->n_host, b->n_out.p, b->ninst*sizeof(double),
+		                       hipMemcpyDeviceToHost, s));
+		// the norms are returned in host memory, so this call waits for them
 		FVX_HIP(hipStreamSynchronize(s));
-		for(int i = 0; i < b->ninst; i++) out[i] = std::sqrt(h[i]);
+		for(int i = 0; i < b->ninst; i++) out[i] = std::sqrt(b->n_host[i]);
 	});
 }
 
@@ -506,7 +519,13 @@ fvx_status fvx_set_launch(fvx_disc d, int kernel, int block, int xcd)
 		        "block must be 64, 128, 256 or 512");
 		require(xcd == 0 || xcd == 1, "xcd must be 0 or 1");
 		if(kernel == FVX_KERNEL_GRADIENTS) { d->tune.grad_block = block; d->tune.grad_xcd = xcd; }
-		else if(kernel == FVX_KERNEL_FLUX) { d->tune.flux_block = block; d->tune.flux_xcd = xcd; }
+		else if(kernel == FVX_KERNEL_FLUX) {
+			d->tune.flux_block = block; d->tune.flux_xcd = xcd; d->tune.flux_tasks = 0;
+		}
+		else if(kernel == FVX_KERNEL_FLUX_TASKS) {
+			require(block == fvx::FT_THREADS, "the face-task flux kernel runs 256 threads per workgroup");
+			d->tune.flux_xcd = xcd; d->tune.flux_tasks = 1;
+		}
 		else if(kernel == FVX_KERNEL_JACOBIAN) {
 			require(block <= 256, "the Jacobian kernel takes at most 256 threads per workgroup");
 			d->tune.jac_block = block; d->tune.jac_xcd = xcd;
@@ -521,6 +540,9 @@ fvx_status fvx_get_launch(fvx_disc d, int kernel, int* block, int* xcd)
 		require(d && block && xcd, "null argument");
 		if(kernel == FVX_KERNEL_GRADIENTS) { *block = d->tune.grad_block; *xcd = d->tune.grad_xcd; }
 		else if(kernel == FVX_KERNEL_FLUX) { *block = d->tune.flux_block; *xcd = d->tune.flux_xcd; }
+		else if(kernel == FVX_KERNEL_FLUX_TASKS) {
+			*block = d->tune.flux_tasks ? fvx::FT_THREADS : 0; *xcd = d->tune.flux_xcd;
+		}
 		else if(kernel == FVX_KERNEL_JACOBIAN) { *block = d->tune.jac_block; *xcd = d->tune.jac_xcd; }
 		else require(false, "kernel has no tunable launch shape");
 	});

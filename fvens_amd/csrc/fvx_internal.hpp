@@ -92,6 +92,12 @@ struct fvx_mesh_s {
 	fvx::DBuf<double> d_centre;            // [nelem][2]
 	fvx::DBuf<double> d_area;              // [nelem]
 	fvx::DBuf<int> d_bhost, d_btag;        // [nbface] host cell and marker of boundary faces
+	// face tasks of the face-task flux kernel: per workgroup of FT_CELLS consecutive cells,
+	// every face touching them once (flux_tasks.hpp)
+	fvx::DBuf<int4> d_ftask;               // {face, left, right or -1-ib, local slots}
+	fvx::DBuf<int> d_ftptr;                // [ngroups + 1] task ranges
+	fvx::DBuf<unsigned char> d_nslot;      // [nelem] faces per cell
+	double ft_internal = 0.0;              // share of interior faces inside one workgroup
 };
 
 namespace fvx {
@@ -114,6 +120,8 @@ struct Profiler {
 enum { PROF_GRADIENTS = 0, PROF_FLUX = 1, PROF_JACOBIAN = 2, PROF_PC = 3 };
 struct SolverWork;
 void destroy_solver_work(SolverWork* w);
+struct Krylov;
+void destroy_krylov(Krylov* k);
 }
 
 namespace fvx {
@@ -122,6 +130,7 @@ namespace fvx {
 struct Tuning {
 	int flux_block = 256, grad_block = 256, jac_block = 256;
 	int flux_xcd = 1, grad_xcd = 1, jac_xcd = 1;
+	int flux_tasks = 1;    ///< face-task flux kernel where it applies (else quad per cell)
 };
 }
 
@@ -143,6 +152,8 @@ struct fvx_disc_s {
 	fvx::DBuf<double> d_grad;              // [nelem][8] unlimited primitive gradients
 	fvx::DBuf<double> d_rgrad;             // [nelem][8] reconstruction gradients (limited)
 	fvx::DBuf<double> d_scratch;           // reductions for functionals
+	fvx::DBuf<double> mf_aux, mf_raux;     // [nelem][4] matrix-free state and residual, first use
+	fvx::DBuf<double> mf_part;             // matrix-free norm partial sums and |x|^2
 	fvx::SolverWork* work = nullptr;       // implicit-solver workspace, created on first use
 	~fvx_disc_s() { fvx::destroy_solver_work(work); }
 };
@@ -158,7 +169,10 @@ struct fvx_batch_s {
 	fvx::DBuf<int> d_all;                  // 0 .. ninst-1
 	fvx::DBuf<double> d_grad, d_rgrad;     // [ninst][nelem][8]
 	fvx::DBuf<double> d_scratch;           // functional partial sums
+	fvx::DBuf<double> n_r, n_part, n_out;  // residual-norm scratch, allocated on first use
+	double* n_host = nullptr;              // pinned host copy of the norms
 	std::vector<double> norms;             // fixed convergence normalisers (empty: none)
+	~fvx_batch_s() { if(n_host) (void)hipHostFree(n_host); }
 };
 
 /// Block matrix on the cell graph: vals[cell][slot][4][4], slot 0 the diagonal block and slot
@@ -197,17 +211,24 @@ struct fvx_pc_s {
 	fvx::DBuf<unsigned char> d_lineinfo;
 	fvx::DBuf<double> v64;
 	fvx::DBuf<float> v32;
+	fvx::Krylov* kry = nullptr;            // FGMRES workspace of fvx_linear_solve, first use
 	double* f64() const { return v64.p; }
 	float* f32() const { return v32.p; }
+	~fvx_pc_s() { fvx::destroy_krylov(kry); }
 };
 
 namespace fvx {
+/// Symmetric Gauss-Seidel / ILU(0) preconditioners, in multicolour or natural (level) order.
+inline bool pc_sgs(int t) { return t == FVX_PC_SGS || t == FVX_PC_SGS_LEVEL; }
+inline bool pc_ilu(int t) { return t == FVX_PC_ILU0 || t == FVX_PC_ILU0_LEVEL; }
+inline bool pc_level(int t) { return t == FVX_PC_SGS_LEVEL || t == FVX_PC_ILU0_LEVEL; }
 void build_mesh(fvx_mesh_s& m, const fvx_mesh_desc& d);
 void rcm_order(const fvx_mesh_desc& d, int* perm);
 void find_lines(const fvx_mesh_desc& d, double threshold, std::vector<std::vector<int>>& lines,
                 std::vector<int>& celline);
 void line_order(const fvx_mesh_desc& d, double threshold, int graph_ordering, int* perm);
 void hilbert_order(const fvx_mesh_desc& d, int* perm);
+void build_face_tasks(fvx_mesh_s& m);
 void build_disc(fvx_disc_s& d);
 GasParams make_gas(const fvx_physics_cfg& p);
 
@@ -216,6 +237,8 @@ GasParams make_gas(const fvx_physics_cfg& p);
 void launch_gradients(const fvx_disc_s& d, const double* u, double* grad, double* rgrad,
                       int conserved, hipStream_t s, const BatchArg& ba = BatchArg(),
                       int nact = 1);
+/// Whether the face-task form of the flux pass covers this discretisation's options.
+bool flux_tasks_apply(const fvx_disc_s& d);
 void launch_residual(const fvx_disc_s& d, const double* u, double* r, int gettimesteps,
                      double* dt, hipStream_t s, const BatchArg& ba = BatchArg(), int nact = 1,
                      double* grad = nullptr, double* rgrad = nullptr);
@@ -253,6 +276,7 @@ void launch_lincomb(long N, const double* Z, long ldz, long sz, int nv, const do
                     long sy, double* x, long sx, const Inst& L, hipStream_t s);
 void launch_resnorm(int n, const double* r, const double* area, double* partial, double* out,
                     long so, const Inst& L, hipStream_t s);
+void launch_zero(long N, double* x, long sx, const Inst& L, hipStream_t s);
 void launch_mf_prep(long N, const double* u, const double* x, long sx, const double* xx,
                     long sn, double eps, double* aux, const Inst& L, hipStream_t s);
 void launch_mf_finish(int n, const double* x, long sx, const double* dtm, const double* ru,
@@ -272,7 +296,7 @@ int matrix_bsr_nnzb(const fvx_mesh_s& m);
 void matfree_apply(const fvx_disc_s& d, const double* u, const double* ru, const double* dtm,
                    const double* x, double* y, double eps, double* aux, double* raux,
                    double* partial, double* xx, hipStream_t s);
-int linear_solve(const fvx_matrix_s& A, const fvx_pc_s& p, const double* b, double* x,
+int linear_solve(const fvx_matrix_s& A, fvx_pc_s& p, const double* b, double* x,
                  int restart, double rtol, hipStream_t s);
 void steady_solve(fvx_disc_s& d, const fvx_solver_cfg& cfg, double* u, fvx_solve_stats& st,
                   hipStream_t s);

@@ -618,6 +618,17 @@ void k_lincomb(long N, const int* __restrict__ ilist, const double* __restrict__
 	}
 }
 
+/// x = 0 over N values per instance (a kernel rather than hipMemsetAsync, so that callers may
+/// capture it in a graph).
+__global__ __launch_bounds__(BLOCK)
+void k_zero(long N, const int* __restrict__ ilist, double* __restrict__ x, long sx)
+{
+	const long I = inst_of(ilist);
+	x += I*sx;
+	const long stride = (long)gridDim.x*blockDim.x;
+	for(long e = (long)blockIdx.x*blockDim.x + threadIdx.x; e < N; e += stride) x[e] = 0.0;
+}
+
 /// partial[b][0] = block part of sum_c r[c][3]^2 area[c] (the reference's energy-residual
 /// norm, src/ode/aodesolver.cpp:516-530).
 __global__ __launch_bounds__(BLOCK)
@@ -755,16 +766,16 @@ static void pc_setup_t(fvx_pc_s& p, const VTA* vals, VTF* fac, const Inst& L, hi
 {
 	const int n = p.m->nelem;
 	const long sa = mstride(p.m);
-	if(p.type == FVX_PC_JACOBI || p.type == FVX_PC_SGS) {
+	if(p.type == FVX_PC_JACOBI || pc_sgs(p.type)) {
 		hipLaunchKernelGGL((k_invert_diag<VTA, VTF>), dim3(thread_blocks(n), L.nact), dim3(BLOCK),
-		                   0, s, n, L.list, vals, fac, sa, p.type == FVX_PC_SGS ? 1 : 0);
+		                   0, s, n, L.list, vals, fac, sa, pc_sgs(p.type) ? 1 : 0);
 		FVX_HIP(hipGetLastError());
 	} else if(p.type == FVX_PC_LINE) {
 		hipLaunchKernelGGL((k_line_factor<VTA, VTF>), dim3(thread_blocks(p.nvert), L.nact),
 		                   dim3(BLOCK), 0, s, p.nvert, p.d_vptr.p, p.d_vcells.p, p.d_lineinfo.p,
 		                   L.list, vals, fac, sa);
 		FVX_HIP(hipGetLastError());
-	} else if(p.type == FVX_PC_ILU0) {
+	} else if(pc_ilu(p.type)) {
 		for(int c = 0; c < p.ncol; c++) {
 			const int cnt = p.coff[c+1] - p.coff[c];
 			if(cnt == 0) continue;
@@ -898,7 +909,7 @@ template <class VTF>
 static void pc_apply_t(const fvx_pc_s& p, const VTF* fac, const double* r, long sr, double* z,
                        long sz, const Inst& L, hipStream_t s)
 {
-	if(p.type == FVX_PC_SGS) {
+	if(pc_sgs(p.type)) {
 		for(int k = 0; k < p.sweeps; k++) sgs_passes(p, fac, r, sr, z, sz, k == 0, L, s);
 		return;
 	}
@@ -980,6 +991,13 @@ void launch_lincomb(long N, const double* Z, long ldz, long sz, int nv, const do
 {
 	hipLaunchKernelGGL(k_lincomb, dim3(stream_blocks(N), L.nact), dim3(BLOCK), 0, s, N, L.list, Z,
 	                   ldz, sz, nv, y, sy, x, sx);
+	FVX_HIP(hipGetLastError());
+}
+
+void launch_zero(long N, double* x, long sx, const Inst& L, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_zero, dim3(stream_blocks(N), L.nact), dim3(BLOCK), 0, s, N, L.list, x,
+	                   sx);
 	FVX_HIP(hipGetLastError());
 }
 

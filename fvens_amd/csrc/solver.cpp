@@ -159,7 +159,7 @@ static void line_vertices(fvx_pc_s& p, const fvx_mesh_s& m)
 
 void pc_init(fvx_pc_s& p, const fvx_mesh_s* m, int type, int prec, int ninst)
 {
-	if(type < FVX_PC_NONE || type > FVX_PC_LINE)
+	if(type < FVX_PC_NONE || type > FVX_PC_ILU0_LEVEL)
 		throw Error(FVX_ERR_INVALID, "unknown preconditioner type");
 	if(prec != FVX_PREC_F64 && prec != FVX_PREC_F32)
 		throw Error(FVX_ERR_INVALID, "unknown preconditioner precision");
@@ -169,10 +169,10 @@ void pc_init(fvx_pc_s& p, const fvx_mesh_s* m, int type, int prec, int ninst)
 	p.ninst = ninst;
 	p.A = nullptr;
 	const int n = m->nelem;
-	// greedy colouring in cell order: the smallest colour unused by lower-numbered neighbours
 	p.colour.assign(n, 0);
 	p.ncol = 1;
 	if(type == FVX_PC_SGS || type == FVX_PC_ILU0) {
+		// greedy colouring in cell order: the smallest colour unused by lower-numbered neighbours
 		for(int c = 0; c < n; c++) {
 			unsigned used = 0;
 			for(int k = 0; k < 4; k++)
@@ -184,6 +184,23 @@ void pc_init(fvx_pc_s& p, const fvx_mesh_s* m, int type, int prec, int ninst)
 			while(used & (1u << col)) col++;
 			p.colour[c] = col;
 			p.ncol = std::max(p.ncol, col + 1);
+		}
+	}
+	if(pc_level(type)) {
+		// wavefronts of the natural order: a cell's level is one more than the highest level
+		// of its lower-numbered neighbours, so the cells of one level are independent and every
+		// lower-numbered neighbour of a cell lies in an earlier level; sweeping the levels in
+		// turn gives the sequential natural-order result (the role of levelSchedule,
+		// src/mesh/ameshutils.cpp:158-196, with dependency levels instead of contiguous ranges)
+		for(int c = 0; c < n; c++) {
+			int lev = 0;
+			for(int k = 0; k < 4; k++)
+				if(interior_slot(*m, c, k)) {
+					const int o = m->esuel[(size_t)c*4 + k];
+					if(o < c) lev = std::max(lev, p.colour[o] + 1);
+				}
+			p.colour[c] = lev;
+			p.ncol = std::max(p.ncol, lev + 1);
 		}
 	}
 	if(type == FVX_PC_LINE) {
@@ -202,7 +219,9 @@ void pc_init(fvx_pc_s& p, const fvx_mesh_s* m, int type, int prec, int ninst)
 			if(interior_slot(*m, c, k) && p.colour[m->esuel[(size_t)c*4 + k]] < p.colour[c])
 				low[c] |= (unsigned char)(1u << k);
 	if(n > 0) {
-		p.d_colour.upload(p.colour);
+		// ILU(0) takes the lower neighbours of a row in (colour, index) order: the multicolour
+		// order's, or for the natural order the index alone
+		p.d_colour.upload(pc_level(type) ? std::vector<int>(n, 0) : p.colour);
 		p.d_order.upload(p.order);
 		p.d_lowmask.upload(low);
 	}
@@ -226,7 +245,7 @@ void pc_set_sweeps(fvx_pc_s& p, int sweeps)
 {
 	if(sweeps < 1 || sweeps > 64) throw Error(FVX_ERR_INVALID, "sweep count must be in [1, 64]");
 	p.sweeps = sweeps;
-	if(sweeps > 1 && p.type != FVX_PC_SGS && p.type != FVX_PC_LINE && p.type != FVX_PC_NONE) {
+	if(sweeps > 1 && !pc_sgs(p.type) && p.type != FVX_PC_LINE && p.type != FVX_PC_NONE) {
 		const size_t N = 4*(size_t)p.m->nelem*p.ninst;
 		if(p.t1.n != N) { p.t1.alloc(N); p.t2.alloc(N); }
 	}
@@ -407,11 +426,14 @@ static void check_restart(int restart)
 		throw Error(FVX_ERR_INVALID, "Krylov restart length must be in [1, 31]");
 }
 
-int linear_solve(const fvx_matrix_s& A, const fvx_pc_s& p, const double* b, double* x,
+void destroy_krylov(Krylov* k) { delete k; }
+
+int linear_solve(const fvx_matrix_s& A, fvx_pc_s& p, const double* b, double* x,
                  int restart, double rtol, hipStream_t s)
 {
 	check_restart(restart);
-	Krylov k;
+	if(!p.kry) p.kry = new Krylov;
+	Krylov& k = *p.kry;
 	k.ensure(4L*A.m->nelem, restart, 1);
 	std::vector<int> its;
 	k.solve([&](const double* xi, long sx, double* yo, long sy, const Inst& L) {
@@ -423,11 +445,11 @@ int linear_solve(const fvx_matrix_s& A, const fvx_pc_s& p, const double* b, doub
 	return its[0];
 }
 
-/// y = dtm x + (r(u + eps x/|x|) - r(u)) |x|/eps for the instances of L; raux holds
-/// raux_len values (all instances) and is cleared before the residual is accumulated.
+/// y = dtm x + (r(u + eps x/|x|) - r(u)) |x|/eps for the instances of L; raux (instance stride
+/// 4 ncell) is cleared for those instances before the residual is accumulated.
 static void matfree_batch(const fvx_disc_s& d, const double* u, const double* ru,
                           const double* dtm, const double* x, long sx, double* y, long sy,
-                          double eps, double* aux, double* raux, size_t raux_len,
+                          double eps, double* aux, double* raux,
                           double* partial, double* xx, long sxx, const Inst& L,
                           const BatchArg& ba, double* grad, double* rgrad, hipStream_t s)
 {
@@ -435,7 +457,7 @@ static void matfree_batch(const fvx_disc_s& d, const double* u, const double* ru
 	const long N = 4L*n;
 	launch_mdot(N, x, 0, sx, 1, x, sx, partial, xx, sxx, L, s);
 	launch_mf_prep(N, u, x, sx, xx, sxx, eps, aux, L, s);
-	FVX_HIP(hipMemsetAsync(raux, 0, raux_len*sizeof(double), s));
+	launch_zero(N, raux, N, L, s);
 	launch_residual(d, aux, raux, 0, nullptr, s, ba, L.nact, grad, rgrad);
 	launch_mf_finish(n, x, sx, dtm, ru, raux, xx, sxx, eps, y, sy, L, s);
 }
@@ -445,7 +467,7 @@ void matfree_apply(const fvx_disc_s& d, const double* u, const double* ru, const
                    double* partial, double* xx, hipStream_t s)
 {
 	const long N = 4L*d.m->nelem;
-	matfree_batch(d, u, ru, dtm, x, N, y, N, eps, aux, raux, N, partial, xx, 0, Inst(),
+	matfree_batch(d, u, ru, dtm, x, N, y, N, eps, aux, raux, partial, xx, 0, Inst(),
 	              BatchArg(), nullptr, nullptr, s);
 }
 
@@ -603,7 +625,7 @@ void steady_core(fvx_disc_s& d, const fvx_solver_cfg& c, double* u, int ninst,
 					BatchArg bb = ba;
 					bb.list = LL.list;
 					matfree_batch(d, u, w.r.p, w.dtm.p, x, sx, y, sy, c.mf_eps, w.aux.p, w.raux.p,
-					              (size_t)N*ninst, K.partial.p, K.scal.p + Krylov::XX,
+					              K.partial.p, K.scal.p + Krylov::XX,
 					              Krylov::NSCAL, LL, bb, grad, rgrad, s);
 				};
 			else

@@ -23,12 +23,30 @@ namespace fvx {
 FVX_DECL_FLUX(0) FVX_DECL_FLUX(1) FVX_DECL_FLUX(2) FVX_DECL_FLUX(3) FVX_DECL_FLUX(4)
 FVX_DECL_FLUX(5) FVX_DECL_FLUX(6)
 #undef FVX_DECL_FLUX
+#define FVX_DECL_FLUXT(ID) void launch_flux_tasks_##ID(int ncell, const double* u, \
+	const double* rgrad, const int4* tasks, const int* tptr, const unsigned char* nslot, \
+	const FaceGeom* fg, const double* centre, const double* area, const int* bcid, \
+	const GasParams* gpa, const BCTable* bta, double* res, double* dt, int getdt, \
+	const BatchArg& ba, int nact, int xcd, hipStream_t s);
+FVX_DECL_FLUXT(0) FVX_DECL_FLUXT(1) FVX_DECL_FLUXT(2) FVX_DECL_FLUXT(3) FVX_DECL_FLUXT(4)
+FVX_DECL_FLUXT(5) FVX_DECL_FLUXT(6)
+#undef FVX_DECL_FLUXT
 
 /// LIM: 0 none, 1 Venkatakrishnan, 2 Barth-Jespersen. CONS: gradients of the conserved
 /// variables (getGradients) instead of the primitive ones. gout receives the unlimited
 /// gradient (may be null when LIM > 0 and it is not needed), rout the limited one.
 /// Two lanes per cell, each owning two of the cell's local faces; the pair combines its
-/// partial sums, bounds and limiter values with one DPP exchange.
+/// partial sums, bounds and limiter values with one DPP exchange. A workgroup owns a range of
+/// consecutive cells and first stages their primitive states in LDS, so a neighbour inside the
+/// range is read from there instead of being fetched and converted again.
+///
+/// The limiter factor of a cell and variable is the minimum over its faces of a ratio N/D with
+/// D >= 0 (Venkatakrishnan: N = dp^2 + 2 dp dm + eps^2, D = dp^2 + dp dm + 2 dm^2 + eps^2;
+/// Barth-Jespersen: N/D = (max or min difference)/dm), starting from 1. The minimum is tracked
+/// as the pair (N, D) by cross-multiplied comparisons and divided once at the end: the factor is
+/// then the rounded quotient of the minimising face, the value the face-by-face division gives
+/// (up to a rounding-level tie between two faces). A face with D = 0 (0/0 in the reference,
+/// ignored by its minimum) never compares smaller.
 template <int LIM, bool CONS>
 __global__ __launch_bounds__(MAX_BLOCK)
 void k_gradients(int ncell, const double* __restrict__ u, const int* __restrict__ nbr,
@@ -48,25 +66,35 @@ void k_gradients(int ncell, const double* __restrict__ u, const int* __restrict_
 	if(gout) gout += I*8L*ncell;
 	if(rout) rout += I*8L*ncell;
 	const int p = threadIdx.x & 1;
-	const int cell = xcd_block(xcd)*(int)(blockDim.x >> 1) + (threadIdx.x >> 1);
-	const bool valid = cell < ncell;
-	const int cc = valid ? cell : 0;
+	const int cpb = (int)(blockDim.x >> 1);
+	const int c0 = xcd_block(xcd)*cpb;
+	const int lc = threadIdx.x >> 1;
+	const int nloc = min(cpb, ncell - c0);
+	const int cell = c0 + lc;
+	const bool valid = lc < nloc;
+	const int cc = valid ? cell : c0;
+	__shared__ double sw[4][MAX_BLOCK/2];
+
 	double ui[4], wi[4];
 	load4(u + 4*cc, ui);
 	if constexpr(CONS) { for(int k = 0; k < 4; k++) wi[k] = ui[k]; }
 	else prim_lean(gp, ui, wi);
+	if(valid && p == 0) for(int k = 0; k < 4; k++) sw[k][lc] = wi[k];
 	const int2 nb2 = reinterpret_cast<const int2*>(nbr)[2*cc + p];
 	const int2 fc2 = reinterpret_cast<const int2*>(face)[2*cc + p];
 	const int nbs[2] = { valid ? nb2.x : SLOT_UNUSED, valid ? nb2.y : SLOT_UNUSED };
 	const int fcs[2] = { fc2.x, fc2.y };
 	const double2* cf = reinterpret_cast<const double2*>(gcoef) + 4*cc + 2*p;
-
-	// both neighbour gathers in flight together (a boundary or unused slot loads the cell
-	// itself as a placeholder)
+	// neighbours outside the workgroup's range: both gathers in flight together (any other
+	// slot loads the cell itself as a placeholder)
+	bool outside[2];
 	double uj[2][4];
 #pragma unroll
-	for(int k = 0; k < 2; k++) load4(u + 4*(nbs[k] >= 0 ? nbs[k] : cc), uj[k]);
-	const double2 c0 = cf[0], c1 = cf[1];
+	for(int k = 0; k < 2; k++) {
+		outside[k] = nbs[k] >= 0 && (unsigned)(nbs[k] - c0) >= (unsigned)nloc;
+		load4(u + 4*(outside[k] ? nbs[k] : cc), uj[k]);
+	}
+	const double2 c0v = cf[0], c1v = cf[1];
 	// face midpoint minus cell centre of the two slots (zero for an unused slot)
 	double2 off[2] = { make_double2(0, 0), make_double2(0, 0) };
 	if constexpr(LIM > 0) {
@@ -74,25 +102,27 @@ void k_gradients(int ncell, const double* __restrict__ u, const int* __restrict_
 		off[0] = so[0];
 		off[1] = so[1];
 	}
-#pragma unroll
-	for(int k = 0; k < 2; k++) {
-		const int nb = nbs[k];
-		if(nb < 0 && nb != SLOT_UNUSED) {
-			const FaceGeom G = fgeom[fcs[k]];
-			const double n[2] = { G.nx, G.ny };
-			ghost_state(gp, bt, bcid[-1 - nb], ui, n, uj[k]);
-		}
-	}
+	__syncthreads();
 
 	double g[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 	double dmax[4] = {0, 0, 0, 0}, dmin[4] = {0, 0, 0, 0};
 #pragma unroll
 	for(int k = 0; k < 2; k++) {
-		if(nbs[k] == SLOT_UNUSED) continue;
-		const double2 c = k == 0 ? c0 : c1;
+		const int nb = nbs[k];
+		if(nb == SLOT_UNUSED) continue;
 		double wj[4];
-		if constexpr(CONS) { for(int v = 0; v < 4; v++) wj[v] = uj[k][v]; }
-		else prim_lean(gp, uj[k], wj);
+		if(nb >= 0 && !outside[k]) {
+			for(int v = 0; v < 4; v++) wj[v] = sw[v][nb - c0];
+		} else {
+			if(nb < 0) {
+				const FaceGeom G = fgeom[fcs[k]];
+				const double n[2] = { G.nx, G.ny };
+				ghost_state(gp, bt, bcid[-1 - nb], ui, n, uj[k]);
+			}
+			if constexpr(CONS) { for(int v = 0; v < 4; v++) wj[v] = uj[k][v]; }
+			else prim_lean(gp, uj[k], wj);
+		}
+		const double2 c = k == 0 ? c0v : c1v;
 		for(int v = 0; v < 4; v++) {
 			const double dw = wj[v] - wi[v];
 			g[2*v] += c.x*dw;
@@ -109,7 +139,7 @@ void k_gradients(int ncell, const double* __restrict__ u, const int* __restrict_
 		double lg[8];
 		for(int v = 0; v < 4; v++) {
 			const double mx = pair_max(dmax[v]), mn = pair_min(dmin[v]);
-			double phi = 1.0;
+			double bn = 1.0, bd = 1.0;          // the smallest ratio so far, as N / D
 			for(int k = 0; k < 2; k++) {
 				if(nbs[k] == SLOT_UNUSED) continue;
 				// face value by linear extrapolation, then its difference from the cell value
@@ -117,18 +147,23 @@ void k_gradients(int ncell, const double* __restrict__ u, const int* __restrict_
 				uf += g[2*v]*off[k].x;
 				uf += g[2*v+1]*off[k].y;
 				const double dm = uf - wi[v];
-				double ph;
+				double pn, pd;
 				if constexpr(LIM == 1) {
 					const double dp = dm < 0 ? mn : mx;
-					ph = (dp*dp + 2*dp*dm + e2)/(dp*dp + dp*dm + 2*dm*dm + e2);
+					pn = dp*dp + 2*dp*dm + e2;
+					pd = dp*dp + dp*dm + 2*dm*dm + e2;
 				} else {
-					if(dm > 0) ph = 1 < mx/dm ? 1.0 : mx/dm;
-					else if(dm < 0) ph = 1 < mn/dm ? 1.0 : mn/dm;
-					else ph = 1.0;
+					if(dm > 0) { pn = mx; pd = dm; }
+					else if(dm < 0) { pn = -mn; pd = -dm; }
+					else { pn = 1.0; pd = 1.0; }
 				}
-				phi = fmin(phi, ph);
+				if(pn*bd < bn*pd) { bn = pn; bd = pd; }
 			}
-			phi = pair_min(phi);
+			// the pair's two candidates, compared the same way on both lanes
+			const double on = dpp_d<DPP_XOR1>(bn), od = dpp_d<DPP_XOR1>(bd);
+			if(p == 0) { if(on*bd < bn*od) { bn = on; bd = od; } }
+			else { if(!(bn*od < on*bd)) { bn = on; bd = od; } }
+			const double phi = bn/bd;
 			lg[2*v] = phi*g[2*v];
 			lg[2*v+1] = phi*g[2*v+1];
 		}
@@ -258,6 +293,13 @@ void launch_gradients(const fvx_disc_s& d, const double* u, double* grad, double
 	d.prof.stop(s);
 }
 
+bool flux_tasks_apply(const fvx_disc_s& d)
+{
+	// linear reconstruction (with or without a limiter or WENO weights), inviscid
+	const bool order2 = d.nc.order2 && d.nc.gradientscheme != FVX_GRAD_NONE;
+	return order2 && d.nc.reconstruction != FVX_RECON_VANALBADA && !d.pc.viscous;
+}
+
 void launch_residual(const fvx_disc_s& d, const double* u, double* r, int gettimesteps,
                      double* dt, hipStream_t s, const BatchArg& ba, int nact, double* grad,
                      double* rgrad)
@@ -287,6 +329,21 @@ void launch_residual(const fvx_disc_s& d, const double* u, double* r, int gettim
 		}
 	}
 	d.prof.start(PROF_FLUX, s);
+	if(d.tune.flux_tasks && flux_tasks_apply(d)) {
+		// face-task form (flux_tasks.hpp): each face evaluated once per workgroup
+		switch(d.nc.conv_numflux) {
+#define FVX_T(ID) case ID: launch_flux_tasks_##ID(m.nelem, u, rg, m.d_ftask.p, m.d_ftptr.p, \
+	m.d_nslot.p, m.d_fgeom.p, m.d_centre.p, m.d_area.p, d.d_bcid.p, \
+	ba.gp ? ba.gp : d.d_gas.p, d.d_bct.p, r, dt, gettimesteps, ba, nact, d.tune.flux_xcd, s); \
+	break;
+		FVX_T(0) FVX_T(1) FVX_T(2) FVX_T(3) FVX_T(4) FVX_T(5) FVX_T(6)
+#undef FVX_T
+		default: throw Error(FVX_ERR_INVALID, "unknown flux");
+		}
+		FVX_HIP(hipGetLastError());
+		d.prof.stop(s);
+		return;
+	}
 	switch(d.nc.conv_numflux) {
 #define FVX_C(ID) case ID: launch_flux_##ID(mode, visc, m.nelem, u, rg, ug, m.d_nbr.p, \
 	m.d_face.p, m.d_fgeom.p, m.d_centre.p, m.d_area.p, d.d_bcid.p, \

@@ -252,7 +252,7 @@ class FlowFV:
         check(lib().fvx_get_primitive_gradients(self.h, _dptr(u, 4 * n), _dptr(grads, 8 * n),
                                                 _stream()))
 
-    KERNELS = {"gradients": 0, "flux": 1, "jacobian": 2, "pc": 3}
+    KERNELS = {"gradients": 0, "flux": 1, "jacobian": 2, "pc": 3, "flux_tasks": 4}
 
     def set_launch(self, kernel, block, xcd=True):
         check(lib().fvx_set_launch(self.h, self.KERNELS[kernel], int(block), int(xcd)))
@@ -364,7 +364,7 @@ PREC = {"f64": 0, "f32": 1}
 
 class TuneRow(C.Structure):
     _fields_ = [("kernel", C.c_int), ("block", C.c_int), ("xcd", C.c_int), ("ms", C.c_double)]
-PC = {"none": 0, "jacobi": 1, "sgs": 2, "ilu0": 3, "line": 4}
+PC = {"none": 0, "jacobi": 1, "sgs": 2, "ilu0": 3, "line": 4, "sgs_level": 5, "ilu0_level": 6}
 
 
 class SolverCfg(C.Structure):

@@ -360,7 +360,9 @@ public:
 		c.min_relax = conf_.min_nl_update;
 		c.pc = detail::lookup(conf_.pc_type, {{"none", FVX_PC_NONE}, {"jacobi", FVX_PC_JACOBI},
 		                                      {"sgs", FVX_PC_SGS}, {"ilu0", FVX_PC_ILU0},
-		                                      {"line", FVX_PC_LINE}},
+		                                      {"line", FVX_PC_LINE},
+		                                      {"sgs_level", FVX_PC_SGS_LEVEL},
+		                                      {"ilu0_level", FVX_PC_ILU0_LEVEL}},
 		                      "preconditioner");
 		c.pc_precision = conf_.pc_precision;
 		c.pc_sweeps = conf_.pc_sweeps;

@@ -170,7 +170,14 @@ enum { FVX_PREC_F64 = 0, FVX_PREC_F32 = 1 };
  * vertices of the reference's line-point graph (src/mesh/meshordering.cpp:264-385); each line's
  * block-tridiagonal system is solved exactly (block Thomas). The reference reaches the same
  * effect through "-mesh_reorder line" followed by PETSc's ILU or SOR in that order. */
-enum { FVX_PC_NONE = 0, FVX_PC_JACOBI = 1, FVX_PC_SGS = 2, FVX_PC_ILU0 = 3, FVX_PC_LINE = 4 };
+/* FVX_PC_SGS_LEVEL and FVX_PC_ILU0_LEVEL: symmetric Gauss-Seidel and block ILU(0) in the natural
+ * (cell) order, the reference's PETSc -pc_type sor and bjacobi + ilu on one rank
+ * (src/ode/aodesolver.cpp:483, testcases/defaults.solverc, testcases/naca0012/opts.solverc). The
+ * sweeps run over wavefront levels (cells whose lower-numbered neighbours all lie in earlier
+ * levels, src/mesh/ameshutils.cpp:158-196), one parallel launch per level; the result is the
+ * sequential natural-order one. fvx_pc_ordering reports the levels as colours. */
+enum { FVX_PC_NONE = 0, FVX_PC_JACOBI = 1, FVX_PC_SGS = 2, FVX_PC_ILU0 = 3, FVX_PC_LINE = 4,
+       FVX_PC_SGS_LEVEL = 5, FVX_PC_ILU0_LEVEL = 6 };
 
 typedef struct fvx_matrix_s* fvx_matrix;
 typedef struct fvx_pc_s* fvx_pc;
@@ -377,9 +384,12 @@ fvx_status fvx_mpnn_train_step(fvx_mpnn n, int ngraph, const float* targets, con
 
 /* ---- timing: the TimingData role of the reference (src/ode/aodesolver.hpp:46-67) ---- */
 
-/* Kernel ids for fvx_profile_read. */
+/* Kernel ids for fvx_profile_read and fvx_set_launch / fvx_get_launch. FVX_KERNEL_FLUX_TASKS
+ * is the face-task form of the face-flux pass (linear reconstruction, inviscid): setting a launch
+ * for it (block 256) selects that form, setting one for FVX_KERNEL_FLUX selects the quad-per-cell
+ * form; both give the same bits. Its time is read as FVX_KERNEL_FLUX. */
 enum { FVX_KERNEL_GRADIENTS = 0, FVX_KERNEL_FLUX = 1, FVX_KERNEL_JACOBIAN = 2,
-       FVX_KERNEL_PC = 3 };
+       FVX_KERNEL_PC = 3, FVX_KERNEL_FLUX_TASKS = 4 };
 
 /* on != 0 resets the counters and starts recording HIP events around every kernel the handle
  * launches (on the launch stream); on == 0 stops recording. */
