@@ -121,7 +121,12 @@ def _solve_case(name):
     return raw, phys, n1, n2, ramps
 
 
-def gpu_time_to_tol(name, tol, precision, sweeps=16):
+PC_NAMES = {"sgs": "multicolour SGS", "ilu0": "multicolour block ILU(0)",
+            "sgs_level": "natural-order SGS", "ilu0_level": "natural-order block ILU(0)",
+            "bjacobi_ilu0": "block Jacobi + ILU(0)"}
+
+
+def gpu_time_to_tol(name, tol, precision, sweeps=16, pc="sgs", subdomain=256):
     """Pseudo-time solve from free stream (first-order starter, then the main solve) until
     sqrt(sum r_E^2 area) <= tol x the same norm at the free-stream state."""
     import numpy as np
@@ -129,17 +134,17 @@ def gpu_time_to_tol(name, tol, precision, sweeps=16):
     from fvens_amd import fvx
     raw, phys, n1, n2, ramps = _solve_case(name)
     mesh = fvx.Mesh(raw)
-    pc = fvx.physics_cfg(**phys)
-    d1 = fvx.FlowFV(mesh, pc, fvx.numerics_cfg(**n1))
-    d2 = fvx.FlowFV(mesh, pc, fvx.numerics_cfg(**n2))
+    phys_cfg = fvx.physics_cfg(**phys)
+    d1 = fvx.FlowFV(mesh, phys_cfg, fvx.numerics_cfg(**n1))
+    d2 = fvx.FlowFV(mesh, phys_cfg, fvx.numerics_cfg(**n2))
     uinf = torch.tensor(np.tile(d1.freestream(), (raw.nelem, 1)), device="cuda")
     r = torch.zeros_like(uinf)
     dt = torch.zeros(raw.nelem, dtype=torch.float64, device="cuda")
     d2.compute_residual(uinf, r, True, dt)
     _, _, area, _ = mesh.faces()
     norm0 = float(np.sqrt((r[:, 3].cpu().numpy() ** 2 * area).sum()))
-    kw = dict(min_relax=0.2, pc="sgs", pc_sweeps=sweeps, pc_precision=precision,
-              jac_precision=precision, raise_on_failure=False)
+    kw = dict(min_relax=0.2, pc=pc, pc_sweeps=sweeps, pc_precision=precision,
+              jac_precision=precision, pc_subdomain_cells=subdomain, raise_on_failure=False)
     u = uinf.clone()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -155,7 +160,9 @@ def gpu_time_to_tol(name, tol, precision, sweeps=16):
     return dict(seconds=round(t2 - t0, 4), main_seconds=round(t2 - t1, 4),
                 starter_steps=s1["steps"], steps=s2["steps"], lin_iters=s2["lin_iters"],
                 final_rel=s2["final_rel"], converged=bool(s2["converged"]),
-                CL=f["CL"], CDp=f["CDp"], precision=precision, pc=f"multicolour SGS x{sweeps}")
+                CL=f["CL"], CDp=f["CDp"], precision=precision,
+                pc=f"{PC_NAMES[pc]} x{sweeps}"
+                   + (f" ({subdomain}-cell subdomains)" if pc == "bjacobi_ilu0" else ""))
 
 
 def viscous_time_to_tol(precision, tol=1e-10, oracle=False):

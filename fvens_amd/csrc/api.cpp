@@ -297,6 +297,24 @@ fvx_status fvx_pc_ordering(fvx_pc p, int* colour, int* ncolours)
 	});
 }
 
+fvx_status fvx_pc_set_subdomain_cells(fvx_pc p, int cells)
+{
+	return guarded([&] {
+		require(p != nullptr, "null handle");
+		fvx::pc_set_subdomain_cells(*p, cells);
+	});
+}
+
+fvx_status fvx_pc_subdomains(fvx_pc p, int* part, int* nparts)
+{
+	return guarded([&] {
+		require(p != nullptr, "null handle");
+		require(p->type == FVX_PC_BJACOBI_ILU0, "the preconditioner has no subdomains");
+		if(part) std::memcpy(part, p->part.data(), sizeof(int)*p->part.size());
+		if(nparts) *nparts = p->nsub;
+	});
+}
+
 fvx_status fvx_linear_solve(fvx_matrix A, fvx_pc p, const double* b, double* x, int restart,
                             double rtol, int* iters, void* stream)
 {
@@ -519,13 +537,7 @@ fvx_status fvx_set_launch(fvx_disc d, int kernel, int block, int xcd)
 		        "block must be 64, 128, 256 or 512");
 		require(xcd == 0 || xcd == 1, "xcd must be 0 or 1");
 		if(kernel == FVX_KERNEL_GRADIENTS) { d->tune.grad_block = block; d->tune.grad_xcd = xcd; }
-		else if(kernel == FVX_KERNEL_FLUX) {
-			d->tune.flux_block = block; d->tune.flux_xcd = xcd; d->tune.flux_tasks = 0;
-		}
-		else if(kernel == FVX_KERNEL_FLUX_TASKS) {
-			require(block == fvx::FT_THREADS, "the face-task flux kernel runs 256 threads per workgroup");
-			d->tune.flux_xcd = xcd; d->tune.flux_tasks = 1;
-		}
+		else if(kernel == FVX_KERNEL_FLUX) { d->tune.flux_block = block; d->tune.flux_xcd = xcd; }
 		else if(kernel == FVX_KERNEL_JACOBIAN) {
 			require(block <= 256, "the Jacobian kernel takes at most 256 threads per workgroup");
 			d->tune.jac_block = block; d->tune.jac_xcd = xcd;
@@ -540,9 +552,6 @@ fvx_status fvx_get_launch(fvx_disc d, int kernel, int* block, int* xcd)
 		require(d && block && xcd, "null argument");
 		if(kernel == FVX_KERNEL_GRADIENTS) { *block = d->tune.grad_block; *xcd = d->tune.grad_xcd; }
 		else if(kernel == FVX_KERNEL_FLUX) { *block = d->tune.flux_block; *xcd = d->tune.flux_xcd; }
-		else if(kernel == FVX_KERNEL_FLUX_TASKS) {
-			*block = d->tune.flux_tasks ? fvx::FT_THREADS : 0; *xcd = d->tune.flux_xcd;
-		}
 		else if(kernel == FVX_KERNEL_JACOBIAN) { *block = d->tune.jac_block; *xcd = d->tune.jac_xcd; }
 		else require(false, "kernel has no tunable launch shape");
 	});

@@ -2,7 +2,6 @@
  * by the build, one object per flux so the instantiations compile in parallel) and exposes a
  * launcher selecting the reconstruction mode and the viscous variant. */
 #include "flux_kernel.hpp"
-#include "flux_tasks.hpp"
 
 #ifndef FVX_FLUX_ID
 #error "FVX_FLUX_ID must be defined"
@@ -38,21 +37,6 @@ void FVX_CAT(launch_flux_, FVX_FLUX_ID)(int mode, bool visc, int ncell, const do
 		else FVX_LAUNCH(2, true);
 	}
 #undef FVX_LAUNCH
-}
-
-void FVX_CAT(launch_flux_tasks_, FVX_FLUX_ID)(int ncell, const double* u, const double* rgrad,
-                                              const int4* tasks, const int* tptr,
-                                              const unsigned char* nslot, const FaceGeom* fg,
-                                              const double* centre, const double* area,
-                                              const int* bcid, const GasParams* gpa,
-                                              const BCTable* bta, double* res, double* dt,
-                                              int getdt, const BatchArg& ba, int nact, int xcd,
-                                              hipStream_t s)
-{
-	const int ngroup = (ncell + FT_CELLS - 1)/FT_CELLS;
-	hipLaunchKernelGGL((k_flux_tasks<FVX_FLUX_ID>), dim3(ngroup, nact), dim3(FT_THREADS), 0, s,
-	                   ncell, u, rgrad, tasks, tptr, nslot, fg, centre, area, bcid, gpa, bta, ba,
-	                   res, dt, getdt, xcd);
 }
 
 }

@@ -106,6 +106,7 @@ void k_flux(int ncell, const double* __restrict__ u, const double* __restrict__ 
 	// instance: state, gradients, residual and time steps are instance-major; gpa holds the
 	// gas parameters of each instance (one entry for a single solve)
 	const long I = ba.list ? ba.list[blockIdx.y] : 0;
+	if(I < 0) return;                     // a retired slot of a compacted list
 	const GasParams& gp = gpa[I];
 	const BCTable& bt = *bta;
 	u += I*4L*ncell;

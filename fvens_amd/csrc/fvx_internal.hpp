@@ -92,12 +92,6 @@ struct fvx_mesh_s {
 	fvx::DBuf<double> d_centre;            // [nelem][2]
 	fvx::DBuf<double> d_area;              // [nelem]
 	fvx::DBuf<int> d_bhost, d_btag;        // [nbface] host cell and marker of boundary faces
-	// face tasks of the face-task flux kernel: per workgroup of FT_CELLS consecutive cells,
-	// every face touching them once (flux_tasks.hpp)
-	fvx::DBuf<int4> d_ftask;               // {face, left, right or -1-ib, local slots}
-	fvx::DBuf<int> d_ftptr;                // [ngroups + 1] task ranges
-	fvx::DBuf<unsigned char> d_nslot;      // [nelem] faces per cell
-	double ft_internal = 0.0;              // share of interior faces inside one workgroup
 };
 
 namespace fvx {
@@ -130,7 +124,6 @@ namespace fvx {
 struct Tuning {
 	int flux_block = 256, grad_block = 256, jac_block = 256;
 	int flux_xcd = 1, grad_xcd = 1, jac_xcd = 1;
-	int flux_tasks = 1;    ///< face-task flux kernel where it applies (else quad per cell)
 };
 }
 
@@ -209,6 +202,15 @@ struct fvx_pc_s {
 	std::vector<int> vptr, vcells, lcnt;   // lcnt: lines among the vertices of each colour
 	fvx::DBuf<int> d_vptr, d_vcells, d_ocell;   // d_ocell: first cell of each vertex in order
 	fvx::DBuf<unsigned char> d_lineinfo;
+	// block Jacobi with ILU(0) per subdomain: subdomain of every cell; the cells listed by
+	// (subdomain, level, index) with subdomain ranges bptr and level boundaries blv (subdomain b
+	// owns blv[blvptr[b] .. blvptr[b+1]]); per listed cell the positions of its neighbours in
+	// its subdomain's list (-1: not in the subdomain); the neighbour slots restricted to the
+	// subdomain (-1 elsewhere)
+	int subdomain_cells = 256, nsub = 0;
+	std::vector<int> part;
+	fvx::DBuf<int> d_bptr, d_blvptr, d_blv, d_bcells, d_bnbr;
+	fvx::DBuf<int4> d_ploc;
 	fvx::DBuf<double> v64;
 	fvx::DBuf<float> v32;
 	fvx::Krylov* kry = nullptr;            // FGMRES workspace of fvx_linear_solve, first use
@@ -222,13 +224,14 @@ namespace fvx {
 inline bool pc_sgs(int t) { return t == FVX_PC_SGS || t == FVX_PC_SGS_LEVEL; }
 inline bool pc_ilu(int t) { return t == FVX_PC_ILU0 || t == FVX_PC_ILU0_LEVEL; }
 inline bool pc_level(int t) { return t == FVX_PC_SGS_LEVEL || t == FVX_PC_ILU0_LEVEL; }
+/// Largest subdomain of FVX_PC_BJACOBI_ILU0 (one workgroup's LDS holds its vector).
+constexpr int BJ_MAX_CELLS = 1024;
 void build_mesh(fvx_mesh_s& m, const fvx_mesh_desc& d);
 void rcm_order(const fvx_mesh_desc& d, int* perm);
 void find_lines(const fvx_mesh_desc& d, double threshold, std::vector<std::vector<int>>& lines,
                 std::vector<int>& celline);
 void line_order(const fvx_mesh_desc& d, double threshold, int graph_ordering, int* perm);
 void hilbert_order(const fvx_mesh_desc& d, int* perm);
-void build_face_tasks(fvx_mesh_s& m);
 void build_disc(fvx_disc_s& d);
 GasParams make_gas(const fvx_physics_cfg& p);
 
@@ -237,8 +240,6 @@ GasParams make_gas(const fvx_physics_cfg& p);
 void launch_gradients(const fvx_disc_s& d, const double* u, double* grad, double* rgrad,
                       int conserved, hipStream_t s, const BatchArg& ba = BatchArg(),
                       int nact = 1);
-/// Whether the face-task form of the flux pass covers this discretisation's options.
-bool flux_tasks_apply(const fvx_disc_s& d);
 void launch_residual(const fvx_disc_s& d, const double* u, double* r, int gettimesteps,
                      double* dt, hipStream_t s, const BatchArg& ba = BatchArg(), int nact = 1,
                      double* grad = nullptr, double* rgrad = nullptr);
@@ -273,7 +274,18 @@ void launch_gs_update(long N, const double* V, long ldv, long sv, int nv, const 
 void launch_scale(long N, const double* b, long sb, const double* nrm2, long sn, double* out,
                   long so, const Inst& L, hipStream_t s);
 void launch_lincomb(long N, const double* Z, long ldz, long sz, int nv, const double* y,
-                    long sy, double* x, long sx, const Inst& L, hipStream_t s);
+                    long sy, double* x, long sx, const Inst& L, hipStream_t s,
+                    const int* nvi = nullptr);
+/// Device least squares of FGMRES (linalg.hip): state doubles per instance and the launchers
+/// of the cycle start, the per-step rotation and the cycle end.
+long fgmres_state_size();
+void launch_fgmres_begin(int nids, const int* ids, const double* scal, long ns, int beta_slot,
+                         double* kst, int* list, int* hcount, hipStream_t s);
+void launch_fgmres_rotate(int j, double rtol, int nids, const int* ids, const double* scal,
+                          long ns, int h_slot, int nrm_slot, double* kst, int* list,
+                          int* hcount, hipStream_t s);
+void launch_fgmres_end(int nids, const int* ids, const double* kst, double* scal, long ns,
+                       int y_slot, int* its, hipStream_t s);
 void launch_resnorm(int n, const double* r, const double* area, double* partial, double* out,
                     long so, const Inst& L, hipStream_t s);
 void launch_zero(long N, double* x, long sx, const Inst& L, hipStream_t s);
@@ -291,6 +303,7 @@ void launch_update_explicit(int n, double* u, const double* r, const double* dt,
 void matrix_init(fvx_matrix_s& A, const fvx_mesh_s* m, int prec, int ninst = 1);
 void pc_init(fvx_pc_s& p, const fvx_mesh_s* m, int type, int prec, int ninst = 1);
 void pc_set_sweeps(fvx_pc_s& p, int sweeps);
+void pc_set_subdomain_cells(fvx_pc_s& p, int cells);
 void matrix_export_bsr(const fvx_matrix_s& A, int* rowptr, int* colind, double* vals);
 int matrix_bsr_nnzb(const fvx_mesh_s& m);
 void matfree_apply(const fvx_disc_s& d, const double* u, const double* ru, const double* dtm,

@@ -246,6 +246,7 @@ void k_jacobian(int ncell, const double* __restrict__ u, const int* __restrict__
 	// instance: state, matrix, time steps and CFL number are per instance; gpa holds the gas
 	// parameters of each instance (one entry for a single solve)
 	const long I = ba.list ? ba.list[blockIdx.y] : 0;
+	if(I < 0) return;                     // a retired slot of a compacted list
 	const GasParams& gp = gpa[I];
 	const BCTable& bt = *bta;
 	const double cfl = ba.cfl ? ba.cfl[I] : cfl0;

@@ -111,7 +111,9 @@ __device__ __forceinline__ int4 load_nbr(const int* nbr, int c)
 }
 
 /* The kernels below take an instance list `ilist` (null: one instance, index 0) and run
- * gridDim.y instances; every per-instance array argument comes with its instance stride. */
+ * gridDim.y instances; every per-instance array argument comes with its instance stride. An
+ * entry -1 is a retired slot of a list compacted on the device (the FGMRES cycle below); its
+ * workgroups return at once. */
 __device__ __forceinline__ long inst_of(const int* ilist)
 {
 	return ilist ? ilist[blockIdx.y] : 0;
@@ -127,6 +129,7 @@ void k_spmv(int n, const int* __restrict__ ilist, const VT* __restrict__ vals, l
             const double* __restrict__ b, long sb, double* __restrict__ y, long sy)
 {
 	const long I = inst_of(ilist);
+	if(I < 0) return;                     // a retired slot of a compacted list
 	vals += I*sa; x += I*sx; y += I*sy;
 	if constexpr(RES) b += I*sb;
 	const int q = threadIdx.x & 3;
@@ -155,6 +158,7 @@ void k_axpy1(long N, const int* __restrict__ ilist, const double* __restrict__ x
              double* __restrict__ y, long sy)
 {
 	const long I = inst_of(ilist);
+	if(I < 0) return;                     // a retired slot of a compacted list
 	x += I*sx; y += I*sy;
 	const long stride = (long)gridDim.x*blockDim.x;
 	for(long e = (long)blockIdx.x*blockDim.x + threadIdx.x; e < N; e += stride) y[e] += x[e];
@@ -185,6 +189,7 @@ void k_invert_diag(int n, const int* __restrict__ ilist, const VTA* __restrict__
                    VTF* __restrict__ fac, long sa, int copy_off)
 {
 	const long I = inst_of(ilist);
+	if(I < 0) return;                     // a retired slot of a compacted list
 	vals += I*sa; fac += I*sa;
 	const int c = blockIdx.x*blockDim.x + threadIdx.x;
 	if(c >= n) return;
@@ -206,6 +211,7 @@ void k_jacobi_apply(int n, const int* __restrict__ ilist, const VTF* __restrict_
                     const double* __restrict__ r, long sr, double* __restrict__ z, long sz)
 {
 	const long I = inst_of(ilist);
+	if(I < 0) return;                     // a retired slot of a compacted list
 	fac += I*sa; r += I*sr; z += I*sz;
 	const int q = threadIdx.x & 3;
 	const int c = xcd_block()*CELLS_PER_BLOCK + (threadIdx.x >> 2);
@@ -227,6 +233,7 @@ void k_sgs_colour(int cnt, const int* __restrict__ list, const int* __restrict__
                   long sr, double* __restrict__ z, long sz)
 {
 	const long I = inst_of(ilist);
+	if(I < 0) return;                     // a retired slot of a compacted list
 	fac += I*sa; r += I*sr; z += I*sz;
 	const int q = threadIdx.x & 3;
 	const int idx = xcd_block()*CELLS_PER_BLOCK + (threadIdx.x >> 2);
@@ -259,6 +266,7 @@ void k_ilu_fwd(int cnt, const int* __restrict__ list, const int* __restrict__ il
                double* __restrict__ z, long sz)
 {
 	const long I = inst_of(ilist);
+	if(I < 0) return;                     // a retired slot of a compacted list
 	fac += I*sa; r += I*sr; z += I*sz;
 	const int q = threadIdx.x & 3;
 	const int idx = xcd_block()*CELLS_PER_BLOCK + (threadIdx.x >> 2);
@@ -288,6 +296,7 @@ void k_ilu_bwd(int cnt, const int* __restrict__ list, const int* __restrict__ il
                const unsigned char* __restrict__ lowmask, double* __restrict__ z, long sz)
 {
 	const long I = inst_of(ilist);
+	if(I < 0) return;                     // a retired slot of a compacted list
 	fac += I*sa; z += I*sz;
 	const int q = threadIdx.x & 3;
 	const int idx = xcd_block()*CELLS_PER_BLOCK + (threadIdx.x >> 2);
@@ -329,6 +338,7 @@ void k_line_factor(int nv, const int* __restrict__ vptr, const int* __restrict__
                    const VTA* __restrict__ vals, VTF* __restrict__ fac, long sa)
 {
 	const long I = inst_of(ilist);
+	if(I < 0) return;                     // a retired slot of a compacted list
 	vals += I*sa; fac += I*sa;
 	const int v = blockIdx.x*blockDim.x + threadIdx.x;
 	if(v >= nv) return;
@@ -373,6 +383,7 @@ void k_line_colour(int cnt, const int* __restrict__ list, const int* __restrict_
                    const double* __restrict__ r, long sr, double* __restrict__ z, long sz)
 {
 	const long I = inst_of(ilist);
+	if(I < 0) return;                     // a retired slot of a compacted list
 	fac += I*sa; r += I*sr; z += I*sz;
 	const int idx = blockIdx.x*blockDim.x + threadIdx.x;
 	if(idx >= cnt) return;
@@ -435,6 +446,7 @@ void k_ilu_factor(int cnt, const int* __restrict__ list, const int* __restrict__
                   const int* __restrict__ colour, int n)
 {
 	const long I = inst_of(ilist);
+	if(I < 0) return;                     // a retired slot of a compacted list
 	vals += I*sa; fac += I*sa;
 	const int idx = blockIdx.x*blockDim.x + threadIdx.x;
 	if(idx >= cnt) return;
@@ -504,6 +516,82 @@ void k_ilu_factor(int cnt, const int* __restrict__ list, const int* __restrict__
 	for(int s = 1; s < 5; s++) put_block<VTF>(fac + (size_t)i*80 + 16*s, A[s]);
 }
 
+/// Block Jacobi with ILU(0) per subdomain, one application z = M^-1 r. Workgroup b holds
+/// subdomain b, whose cells bcells[bptr[b] .. bptr[b+1]) are listed by (level, index); its
+/// level boundaries are blv[blvptr[b] .. blvptr[b+1]]. ploc gives, per listed cell and
+/// neighbour slot, the neighbour's position in the subdomain's list (-1 outside it), so a
+/// position below the cell's own is a lower-numbered neighbour (an earlier level) and one
+/// above it a higher-numbered one. The vector of the subdomain stays in LDS: the forward
+/// substitution y_i = r_i - sum_{lower k} L_ik y_k runs level by level upwards, the backward
+/// substitution z_i = U_ii^-1 (y_i - sum_{upper j} U_ij z_j) level by level downwards, one
+/// quad per cell (lane q computes row q), a workgroup barrier between levels. The factors are
+/// those of k_ilu_factor on the subdomain-restricted pattern: L_ik in the lower slots, U_ij in
+/// the upper slots, inv(U_ii) in slot 0.
+template <class VTF>
+__global__ __launch_bounds__(BLOCK)
+void k_bj_ilu_apply(const int* __restrict__ bptr, const int* __restrict__ blvptr,
+                    const int* __restrict__ blv, const int* __restrict__ bcells,
+                    const int4* __restrict__ ploc, const int* __restrict__ ilist,
+                    const VTF* __restrict__ fac, long sa, const double* __restrict__ r, long sr,
+                    double* __restrict__ z, long sz)
+{
+	const long I = inst_of(ilist);
+	if(I < 0) return;                     // a retired slot of a compacted list
+	fac += I*sa; r += I*sr; z += I*sz;
+	const int b = blockIdx.x;
+	const int c0 = bptr[b];
+	const int l0 = blvptr[b], nl = blvptr[b+1] - l0;
+	__shared__ double zs[BJ_MAX_CELLS*4];
+	const int q = threadIdx.x & 3, qd = threadIdx.x >> 2;
+	constexpr int NQ = BLOCK/4;
+	for(int l = 0; l < nl; l++) {
+		const int e0 = blv[l0 + l], e1 = blv[l0 + l + 1];
+		for(int idx = e0 + qd; idx < e1; idx += NQ) {
+			const int i = bcells[idx];
+			const int p = idx - c0;
+			const int4 lp4 = ploc[idx];
+			const int lp[4] = { lp4.x, lp4.y, lp4.z, lp4.w };
+			const VTF* blk = fac + (size_t)i*80 + 4*q;
+			double t = r[4*i + q];
+			for(int k = 0; k < 4; k++) {
+				if(lp[k] < 0 || lp[k] >= p) continue;
+				double a[4];
+				load_row<VTF>(blk + 16*(1 + k), a);
+				const double* y = zs + 4*lp[k];
+				t -= a[0]*y[0] + a[1]*y[1] + a[2]*y[2] + a[3]*y[3];
+			}
+			zs[4*p + q] = t;
+		}
+		__syncthreads();
+	}
+	for(int l = nl - 1; l >= 0; l--) {
+		const int e0 = blv[l0 + l], e1 = blv[l0 + l + 1];
+		for(int idx = e0 + qd; idx < e1; idx += NQ) {
+			const int i = bcells[idx];
+			const int p = idx - c0;
+			const int4 lp4 = ploc[idx];
+			const int lp[4] = { lp4.x, lp4.y, lp4.z, lp4.w };
+			const VTF* blk = fac + (size_t)i*80 + 4*q;
+			double t = zs[4*p + q];
+			for(int k = 0; k < 4; k++) {
+				if(lp[k] <= p) continue;
+				double a[4];
+				load_row<VTF>(blk + 16*(1 + k), a);
+				const double* y = zs + 4*lp[k];
+				t -= a[0]*y[0] + a[1]*y[1] + a[2]*y[2] + a[3]*y[3];
+			}
+			const double tv[4] = { quad_bcast<0>(t), quad_bcast<1>(t), quad_bcast<2>(t),
+			                       quad_bcast<3>(t) };
+			double a[4];
+			load_row<VTF>(blk, a);
+			const double v = dot4(a, tv);
+			zs[4*p + q] = v;
+			z[4*i + q] = v;
+		}
+		__syncthreads();
+	}
+}
+
 /* ------------------------------------------------------------- reductions --------------- */
 
 constexpr int RED_BLOCKS = 512;
@@ -527,6 +615,7 @@ void k_mdot(long N, const int* __restrict__ ilist, const double* __restrict__ V,
             long sv, int nv, const double* __restrict__ w, long sw, double* __restrict__ partial)
 {
 	const long I = inst_of(ilist);
+	if(I < 0) return;                     // a retired slot of a compacted list
 	V += I*sv; w += I*sw; partial += I*PART;
 	__shared__ double sh[4];
 	double acc[MAX_DOTS];
@@ -550,6 +639,7 @@ void k_reduce(int nblk, const int* __restrict__ ilist, const double* __restrict_
               int nv, double* __restrict__ out, long so)
 {
 	const long I = inst_of(ilist);
+	if(I < 0) return;                     // a retired slot of a compacted list
 	partial += I*PART; out += I*so;
 	__shared__ double sh[4];
 	for(int i = 0; i < nv; i++) {
@@ -568,6 +658,7 @@ void k_gs_update(long N, const int* __restrict__ ilist, const double* __restrict
                  long sw, double* __restrict__ partial)
 {
 	const long I = inst_of(ilist);
+	if(I < 0) return;                     // a retired slot of a compacted list
 	V += I*sv; h += I*sh_; w += I*sw; partial += I*PART;
 	__shared__ double sh[4];
 	__shared__ double hs[MAX_DOTS];
@@ -592,19 +683,22 @@ void k_scale(long N, const int* __restrict__ ilist, const double* b, long sb,
              const double* __restrict__ nrm2, long sn, double* out, long so)
 {
 	const long I = inst_of(ilist);
+	if(I < 0) return;                     // a retired slot of a compacted list
 	b += I*sb; nrm2 += I*sn; out += I*so;
 	const double h = sqrt(nrm2[0]);
 	const long stride = (long)gridDim.x*blockDim.x;
 	for(long e = (long)blockIdx.x*blockDim.x + threadIdx.x; e < N; e += stride) out[e] = b[e]/h;
 }
 
-/// x = sum_k y_k Z_k, k < nv
+/// x = sum_k y_k Z_k, k < nv (with nvi: nvi[I] terms for instance I; none when negative)
 __global__ __launch_bounds__(BLOCK)
 void k_lincomb(long N, const int* __restrict__ ilist, const double* __restrict__ Z, long ldz,
-               long sz, int nv, const double* __restrict__ y, long sy, double* __restrict__ x,
-               long sx)
+               long sz, int nv, const int* __restrict__ nvi, const double* __restrict__ y,
+               long sy, double* __restrict__ x, long sx)
 {
 	const long I = inst_of(ilist);
+	if(I < 0) return;                     // a retired slot of a compacted list
+	if(nvi) nv = nvi[I];
 	Z += I*sz; y += I*sy; x += I*sx;
 	__shared__ double ys[MAX_DOTS];
 	if(threadIdx.x < MAX_DOTS) ys[threadIdx.x] = threadIdx.x < nv ? y[threadIdx.x] : 0.0;
@@ -624,6 +718,7 @@ __global__ __launch_bounds__(BLOCK)
 void k_zero(long N, const int* __restrict__ ilist, double* __restrict__ x, long sx)
 {
 	const long I = inst_of(ilist);
+	if(I < 0) return;                     // a retired slot of a compacted list
 	x += I*sx;
 	const long stride = (long)gridDim.x*blockDim.x;
 	for(long e = (long)blockIdx.x*blockDim.x + threadIdx.x; e < N; e += stride) x[e] = 0.0;
@@ -636,6 +731,7 @@ void k_resnorm(int n, const int* __restrict__ ilist, const double* __restrict__ 
                const double* __restrict__ area, double* __restrict__ partial)
 {
 	const long I = inst_of(ilist);
+	if(I < 0) return;                     // a retired slot of a compacted list
 	r += I*sr; partial += I*PART;
 	__shared__ double sh[4];
 	double acc = 0.0;
@@ -656,6 +752,7 @@ void k_mf_prep(long N, const int* __restrict__ ilist, const double* __restrict__
                double eps, double* __restrict__ aux, long sa)
 {
 	const long I = inst_of(ilist);
+	if(I < 0) return;                     // a retired slot of a compacted list
 	u += I*su; x += I*sx; xx += I*sn; aux += I*sa;
 	const double pert = eps/sqrt(xx[0]);
 	const long stride = (long)gridDim.x*blockDim.x;
@@ -671,6 +768,7 @@ void k_mf_finish(int n, const int* __restrict__ ilist, const double* __restrict_
                  double eps, double* __restrict__ y, long sy)
 {
 	const long I = inst_of(ilist);
+	if(I < 0) return;                     // a retired slot of a compacted list
 	x += I*sx; dtm += I*(long)n; ru += I*4L*n; raux += I*4L*n; xx += I*sn; y += I*sy;
 	const double pert = eps/sqrt(xx[0]);
 	for(int e = blockIdx.x*blockDim.x + threadIdx.x; e < 4*n; e += gridDim.x*blockDim.x)
@@ -687,6 +785,7 @@ void k_update_implicit(int n, const int* __restrict__ ilist, double* __restrict_
                        const double* __restrict__ du, int full, double min_relax, double gamma)
 {
 	const long I = inst_of(ilist);
+	if(I < 0) return;                     // a retired slot of a compacted list
 	u += I*4L*n; du += I*4L*n;
 	const int c = blockIdx.x*blockDim.x + threadIdx.x;
 	if(c >= n) return;
@@ -716,6 +815,7 @@ void k_update_explicit(int n, const int* __restrict__ ilist, double* __restrict_
                        const double* __restrict__ area, double cfl)
 {
 	const long I = inst_of(ilist);
+	if(I < 0) return;                     // a retired slot of a compacted list
 	u += I*4L*n; r += I*4L*n; dt += I*(long)n;
 	const int c = blockIdx.x*blockDim.x + threadIdx.x;
 	if(c >= n) return;
@@ -775,13 +875,15 @@ static void pc_setup_t(fvx_pc_s& p, const VTA* vals, VTF* fac, const Inst& L, hi
 		                   dim3(BLOCK), 0, s, p.nvert, p.d_vptr.p, p.d_vcells.p, p.d_lineinfo.p,
 		                   L.list, vals, fac, sa);
 		FVX_HIP(hipGetLastError());
-	} else if(pc_ilu(p.type)) {
+	} else if(pc_ilu(p.type) || p.type == FVX_PC_BJACOBI_ILU0) {
+		// block Jacobi: the same factorisation on the pattern restricted to each subdomain
+		const int* nbr = p.type == FVX_PC_BJACOBI_ILU0 ? p.d_bnbr.p : p.m->d_nbr.p;
 		for(int c = 0; c < p.ncol; c++) {
 			const int cnt = p.coff[c+1] - p.coff[c];
 			if(cnt == 0) continue;
 			hipLaunchKernelGGL((k_ilu_factor<VTA, VTF>), dim3(thread_blocks(cnt), L.nact),
 			                   dim3(BLOCK), 0, s, cnt, p.d_order.p + p.coff[c], L.list, vals, fac,
-			                   sa, p.m->d_nbr.p, p.d_lowmask.p, p.d_colour.p, n);
+			                   sa, nbr, p.d_lowmask.p, p.d_colour.p, n);
 			FVX_HIP(hipGetLastError());
 		}
 	}
@@ -886,6 +988,14 @@ static void pc_once(const fvx_pc_s& p, const VTF* fac, const double* r, long sr,
 		FVX_HIP(hipGetLastError());
 		return;
 	}
+	if(p.type == FVX_PC_BJACOBI_ILU0) {
+		if(p.nsub > 0)
+			hipLaunchKernelGGL(k_bj_ilu_apply<VTF>, dim3(p.nsub, L.nact), dim3(BLOCK), 0, s,
+			                   p.d_bptr.p, p.d_blvptr.p, p.d_blv.p, p.d_bcells.p, p.d_ploc.p,
+			                   L.list, fac, sa, r, sr, z, sz);
+		FVX_HIP(hipGetLastError());
+		return;
+	}
 	for(int c = 0; c < p.ncol; c++) {
 		const int cnt = p.coff[c+1] - p.coff[c];
 		if(cnt == 0) continue;
@@ -987,10 +1097,139 @@ void launch_scale(long N, const double* b, long sb, const double* nrm2, long sn,
 }
 
 void launch_lincomb(long N, const double* Z, long ldz, long sz, int nv, const double* y,
-                    long sy, double* x, long sx, const Inst& L, hipStream_t s)
+                    long sy, double* x, long sx, const Inst& L, hipStream_t s, const int* nvi)
 {
 	hipLaunchKernelGGL(k_lincomb, dim3(stream_blocks(N), L.nact), dim3(BLOCK), 0, s, N, L.list, Z,
-	                   ldz, sz, nv, y, sy, x, sx);
+	                   ldz, sz, nv, nvi, y, sy, x, sx);
+	FVX_HIP(hipGetLastError());
+}
+
+/* ------------------------------------------------- FGMRES least-squares on the device ----- */
+
+/* State of one FGMRES cycle per instance (KST doubles, instance-major): the Givens-rotated
+ * Hessenberg columns R[r][j] (r <= j), the rotations cs, sn, the rotated right-hand side g,
+ * beta = |b|, the iteration count and the status (1 running, 0 stopped, 2 zero right-hand
+ * side, -1 non-finite). The kernels below run in one workgroup: thread t serves instance
+ * ids[t], t < nids; thread 0 then writes the running instances, in the order of ids, to the
+ * front of `list` and -1 behind them (the compacted list the next iteration's launches
+ * read), and their number to *hcount (host memory, read by the host after the kernel). */
+constexpr int KR = 0, KCS = MAX_DOTS*MAX_DOTS, KSN = KCS + MAX_DOTS, KG = KSN + MAX_DOTS;
+constexpr int KBETA = KG + MAX_DOTS + 1, KIT = KBETA + 1, KSTAT = KIT + 1;
+constexpr long KST = KSTAT + 1;
+
+__device__ __forceinline__ void fgmres_compact(int nids, const int* __restrict__ ids,
+                                               const double* __restrict__ kst,
+                                               int* __restrict__ list, int* hcount)
+{
+	__syncthreads();
+	if(threadIdx.x != 0) return;
+	int k = 0;
+	for(int t = 0; t < nids; t++)
+		if(kst[ids[t]*KST + KSTAT] == 1.0) list[k++] = ids[t];
+	for(int t = k; t < nids; t++) list[t] = -1;
+	if(hcount) __hip_atomic_store(hcount, k, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_SYSTEM);
+}
+
+/// Cycle start: beta = sqrt(scal[I][beta_slot]), g = (beta, 0, ...), status.
+__global__ __launch_bounds__(BLOCK)
+void k_fgmres_begin(int nids, const int* __restrict__ ids, const double* __restrict__ scal,
+                    long ns, int beta_slot, double* __restrict__ kst, int* __restrict__ list,
+                    int* hcount)
+{
+	for(int t = threadIdx.x; t < nids; t += blockDim.x) {
+		const long I = ids[t];
+		double* st = kst + I*KST;
+		const double beta = sqrt(scal[I*ns + beta_slot]);
+		st[KBETA] = beta;
+		st[KG] = beta;
+		st[KIT] = 0.0;
+		st[KSTAT] = !isfinite(beta) ? -1.0 : beta == 0.0 ? 2.0 : 1.0;
+	}
+	fgmres_compact(nids, ids, kst, list, hcount);
+}
+
+/// Iteration j: the new Hessenberg column h_0..h_j (scal[I][h_slot + r]) and |w|^2
+/// (scal[I][nrm_slot]) go through the previous rotations and a new one; an instance stops when
+/// |g_{j+1}| <= rtol beta or at an exact breakdown, and is marked non-finite on a non-finite
+/// norm.
+__global__ __launch_bounds__(BLOCK)
+void k_fgmres_rotate(int j, double rtol, int nids, const int* __restrict__ ids,
+                     const double* __restrict__ scal, long ns, int h_slot, int nrm_slot,
+                     double* __restrict__ kst, int* __restrict__ list, int* hcount)
+{
+	for(int t = threadIdx.x; t < nids; t += blockDim.x) {
+		const long I = ids[t];
+		double* st = kst + I*KST;
+		if(st[KSTAT] != 1.0) continue;
+		const double* h = scal + I*ns + h_slot;
+		const double hn = sqrt(scal[I*ns + nrm_slot]);
+		if(!isfinite(hn)) { st[KSTAT] = -1.0; continue; }
+		double* cs = st + KCS;
+		double* sn = st + KSN;
+		double* g = st + KG;
+		double hr = h[0];
+		for(int r = 0; r < j; r++) {
+			const double hr1 = h[r + 1];
+			st[KR + r*MAX_DOTS + j] = cs[r]*hr + sn[r]*hr1;
+			hr = -sn[r]*hr + cs[r]*hr1;
+		}
+		const double den = sqrt(hr*hr + hn*hn);
+		cs[j] = hr/den;
+		sn[j] = hn/den;
+		st[KR + j*MAX_DOTS + j] = den;
+		g[j + 1] = -sn[j]*g[j];
+		g[j] = cs[j]*g[j];
+		st[KIT] = j + 1;
+		if(fabs(g[j + 1]) <= rtol*st[KBETA] || hn == 0.0) st[KSTAT] = 0.0;
+	}
+	fgmres_compact(nids, ids, kst, list, hcount);
+}
+
+/// Cycle end: the weights y of the instances' bases (back substitution of R y = g) into
+/// scal[I][y_slot ..], and the iteration count into its[I] (-1: non-finite).
+__global__ __launch_bounds__(BLOCK)
+void k_fgmres_end(int nids, const int* __restrict__ ids, const double* __restrict__ kst,
+                  double* __restrict__ scal, long ns, int y_slot, int* __restrict__ its)
+{
+	for(int t = threadIdx.x; t < nids; t += blockDim.x) {
+		const long I = ids[t];
+		const double* st = kst + I*KST;
+		if(st[KSTAT] == -1.0) { its[I] = -1; continue; }
+		const int it = (int)st[KIT];
+		double* y = scal + I*ns + y_slot;
+		for(int r = it - 1; r >= 0; r--) {
+			double a = st[KG + r];
+			for(int k = r + 1; k < it; k++) a -= st[KR + r*MAX_DOTS + k]*y[k];
+			y[r] = a/st[KR + r*MAX_DOTS + r];
+		}
+		its[I] = it;
+	}
+}
+
+long fgmres_state_size() { return KST; }
+
+void launch_fgmres_begin(int nids, const int* ids, const double* scal, long ns, int beta_slot,
+                         double* kst, int* list, int* hcount, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_fgmres_begin, dim3(1), dim3(BLOCK), 0, s, nids, ids, scal, ns,
+	                   beta_slot, kst, list, hcount);
+	FVX_HIP(hipGetLastError());
+}
+
+void launch_fgmres_rotate(int j, double rtol, int nids, const int* ids, const double* scal,
+                          long ns, int h_slot, int nrm_slot, double* kst, int* list,
+                          int* hcount, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_fgmres_rotate, dim3(1), dim3(BLOCK), 0, s, j, rtol, nids, ids, scal, ns,
+	                   h_slot, nrm_slot, kst, list, hcount);
+	FVX_HIP(hipGetLastError());
+}
+
+void launch_fgmres_end(int nids, const int* ids, const double* kst, double* scal, long ns,
+                       int y_slot, int* its, hipStream_t s)
+{
+	hipLaunchKernelGGL(k_fgmres_end, dim3(1), dim3(BLOCK), 0, s, nids, ids, kst, scal, ns,
+	                   y_slot, its);
 	FVX_HIP(hipGetLastError());
 }
 

@@ -224,54 +224,6 @@ void build_mesh(fvx_mesh_s& m, const fvx_mesh_desc& d)
 	m.d_area.upload(m.area);
 	m.d_bhost.upload(host);
 	m.d_btag.upload(m.btag);
-	build_face_tasks(m);
-}
-
-/* Face tasks of the face-task flux kernel (flux_tasks.hpp). For workgroup b (cells
- * [b FT_CELLS, (b+1) FT_CELLS)), in cell order and local face order: a boundary face, a face
- * to a cell of another workgroup, and a face to a higher-numbered cell of the same workgroup
- * each become one task; the local slot of a cell's face k is k FT_CELLS + (cell - first cell). */
-void build_face_tasks(fvx_mesh_s& m)
-{
-	const int n = m.nelem;
-	const int ng = (n + FT_CELLS - 1)/FT_CELLS;
-	std::vector<int4> tasks;
-	tasks.reserve((size_t)n*2 + 64);
-	std::vector<int> tptr(ng + 1, 0);
-	std::vector<unsigned char> nslot(n);
-	long internal = 0;
-	for(int b = 0; b < ng; b++) {
-		const int c0 = b*FT_CELLS, c1 = std::min(n, c0 + FT_CELLS);
-		for(int c = c0; c < c1; c++) {
-			nslot[c] = (unsigned char)m.nnode[c];
-			for(int k = 0; k < m.nnode[c]; k++) {
-				const int o = m.esuel[(size_t)c*4+k];
-				const int f = m.elemface[(size_t)c*4+k];
-				const unsigned mine = (unsigned)(k*FT_CELLS + (c - c0));
-				if(o >= n) {
-					tasks.push_back(make_int4(f, c, -1 - (o - n), (int)(mine | 0xFFFF0000u)));
-				} else if(o >= c0 && o < c1) {
-					if(o < c) continue;                  // added from the lower cell
-					int ko = -1;
-					for(int j = 0; j < m.nnode[o]; j++)
-						if(m.elemface[(size_t)o*4+j] == f) ko = j;
-					if(ko < 0) throw Error(FVX_ERR_INVALID, "face missing from its right cell");
-					const unsigned theirs = (unsigned)(ko*FT_CELLS + (o - c0));
-					tasks.push_back(make_int4(f, c, o, (int)(mine | (theirs << 16))));
-					internal++;
-				} else if(c < o) {
-					tasks.push_back(make_int4(f, c, o, (int)(mine | 0xFFFF0000u)));
-				} else {
-					tasks.push_back(make_int4(f, o, c, (int)(0xFFFFu | (mine << 16))));
-				}
-			}
-		}
-		tptr[b + 1] = (int)tasks.size();
-	}
-	m.ft_internal = m.ninface > 0 ? (double)internal/m.ninface : 0.0;
-	m.d_ftask.upload(tasks);
-	m.d_ftptr.upload(tptr);
-	m.d_nslot.upload(nslot);
 }
 
 /* Lines of strongly coupled cells (findLines, src/mesh/meshordering.cpp:199-262). Each cell's

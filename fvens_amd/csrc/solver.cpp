@@ -15,6 +15,7 @@
 #include <cmath>
 #include <cstring>
 #include <functional>
+#include <numeric>
 
 namespace fvx {
 
@@ -157,9 +158,151 @@ static void line_vertices(fvx_pc_s& p, const fvx_mesh_s& m)
 	p.d_lineinfo.upload(info);
 }
 
+/* Subdomains of block Jacobi + ILU(0): grown breadth-first over the cell graph from the
+ * lowest-numbered unassigned cell until they hold p.subdomain_cells cells or their connected
+ * region is used up; a piece smaller than half that size is then merged into the adjacent
+ * subdomain found first, if the union holds at most twice that size and at most BJ_MAX_CELLS
+ * cells. Inside a subdomain the natural
+ * order applies: a cell's level (p.colour) is one more than the highest level of its
+ * lower-numbered neighbours in the same subdomain. Fills the device lists of the fused
+ * substitution kernel and, in p.order / p.coff, the cells by (level, index) for the
+ * factorisation launches. */
+static void bj_subdomains(fvx_pc_s& p, const fvx_mesh_s& m)
+{
+	const int n = m.nelem, P = p.subdomain_cells;
+	std::vector<int> part(n, -1), size;
+	std::vector<int> queue;
+	queue.reserve(P);
+	for(int seed = 0; seed < n; seed++) {
+		if(part[seed] >= 0) continue;
+		const int id = (int)size.size();
+		queue.assign(1, seed);
+		part[seed] = id;
+		for(size_t h = 0; h < queue.size() && (int)queue.size() < P; h++) {
+			const int c = queue[h];
+			for(int k = 0; k < 4 && (int)queue.size() < P; k++) {
+				if(!interior_slot(m, c, k)) continue;
+				const int o = m.esuel[(size_t)c*4 + k];
+				if(part[o] >= 0) continue;
+				part[o] = id;
+				queue.push_back(o);
+			}
+		}
+		size.push_back((int)queue.size());
+	}
+	// merge small pieces into a neighbour (first adjacent subdomain in cell and slot order)
+	const int ns0 = (int)size.size();
+	std::vector<int> target(ns0);
+	std::iota(target.begin(), target.end(), 0);
+	std::vector<char> merged(ns0, 0);
+	for(int c = 0; c < n; c++) {
+		const int a = part[c];
+		if(merged[a] || size[a] >= P/2) continue;
+		for(int k = 0; k < 4; k++) {
+			if(!interior_slot(m, c, k)) continue;
+			int b = part[m.esuel[(size_t)c*4 + k]];
+			while(target[b] != b) b = target[b];
+			if(b == a || size[b] + size[a] > std::min(BJ_MAX_CELLS, 2*P)) continue;
+			target[a] = b;
+			size[b] += size[a];
+			merged[a] = 1;
+			break;
+		}
+	}
+	std::vector<int> renum(ns0, -1);
+	int ns = 0;
+	for(int a = 0; a < ns0; a++) {
+		int b = a;
+		while(target[b] != b) b = target[b];
+		if(renum[b] < 0) renum[b] = ns++;
+		renum[a] = renum[b];
+	}
+	for(int c = 0; c < n; c++) part[c] = renum[part[c]];
+	p.part = part;
+	p.nsub = ns;
+	auto same = [&](int c, int k) {
+		return interior_slot(m, c, k) && part[m.esuel[(size_t)c*4 + k]] == part[c];
+	};
+	p.ncol = 1;
+	std::vector<unsigned char> low(n, 0);
+	std::vector<int> bnbr((size_t)n*4, -1);
+	for(int c = 0; c < n; c++) {
+		int lev = 0;
+		for(int k = 0; k < 4; k++) {
+			if(!same(c, k)) continue;
+			const int o = m.esuel[(size_t)c*4 + k];
+			bnbr[(size_t)c*4 + k] = o;
+			if(o < c) { lev = std::max(lev, p.colour[o] + 1); low[c] |= (unsigned char)(1u << k); }
+		}
+		p.colour[c] = lev;
+		p.ncol = std::max(p.ncol, lev + 1);
+	}
+	// factorisation order: by level, then index
+	p.coff.assign(p.ncol + 1, 0);
+	for(int c = 0; c < n; c++) p.coff[p.colour[c] + 1]++;
+	for(int k = 0; k < p.ncol; k++) p.coff[k+1] += p.coff[k];
+	p.order.assign(n, 0);
+	{
+		std::vector<int> fill(p.coff.begin(), p.coff.end() - 1);
+		for(int c = 0; c < n; c++) p.order[fill[p.colour[c]]++] = c;
+	}
+	// substitution lists: by (subdomain, level, index)
+	std::vector<int> cells(n);
+	std::iota(cells.begin(), cells.end(), 0);
+	std::stable_sort(cells.begin(), cells.end(), [&](int a, int b) {
+		return part[a] != part[b] ? part[a] < part[b] : p.colour[a] < p.colour[b]; });
+	std::vector<int> bptr(ns + 1, 0), blvptr(ns + 1, 0), blv, pos(n);
+	for(int i = 0; i < n; i++) {
+		const int c = cells[i], b = part[c];
+		if(i == 0 || part[cells[i-1]] != b) {
+			bptr[b] = i;
+			blvptr[b] = (int)blv.size();
+		}
+		if(i == 0 || part[cells[i-1]] != b || p.colour[cells[i-1]] != p.colour[c])
+			blv.push_back(i);
+		pos[c] = i - bptr[b];
+	}
+	bptr[ns] = n;
+	blv.push_back(n);
+	blvptr[ns] = (int)blv.size() - 1;
+	std::vector<int4> ploc(n);
+	for(int i = 0; i < n; i++) {
+		const int c = cells[i];
+		int l[4];
+		for(int k = 0; k < 4; k++) l[k] = same(c, k) ? pos[m.esuel[(size_t)c*4 + k]] : -1;
+		ploc[i] = make_int4(l[0], l[1], l[2], l[3]);
+	}
+	for(int b = 0; b < ns; b++)
+		if(bptr[b+1] - bptr[b] > BJ_MAX_CELLS)
+			throw Error(FVX_ERR_INVALID, "subdomain larger than a workgroup can hold");
+	if(n > 0) {
+		p.d_bptr.upload(bptr);
+		p.d_blvptr.upload(blvptr);
+		p.d_blv.upload(blv);
+		p.d_bcells.upload(cells);
+		p.d_ploc.upload(ploc);
+		p.d_bnbr.upload(bnbr);
+		p.d_colour.upload(std::vector<int>(n, 0));
+		p.d_order.upload(p.order);
+		p.d_lowmask.upload(low);
+	}
+}
+
+void pc_set_subdomain_cells(fvx_pc_s& p, int cells)
+{
+	if(cells < 16 || cells > BJ_MAX_CELLS)
+		throw Error(FVX_ERR_INVALID, "subdomain size must be in [16, 1024] cells");
+	p.subdomain_cells = cells;
+	if(p.type == FVX_PC_BJACOBI_ILU0 && p.m) {
+		p.colour.assign(p.m->nelem, 0);
+		bj_subdomains(p, *p.m);
+		p.A = nullptr;
+	}
+}
+
 void pc_init(fvx_pc_s& p, const fvx_mesh_s* m, int type, int prec, int ninst)
 {
-	if(type < FVX_PC_NONE || type > FVX_PC_ILU0_LEVEL)
+	if(type < FVX_PC_NONE || type > FVX_PC_BJACOBI_ILU0)
 		throw Error(FVX_ERR_INVALID, "unknown preconditioner type");
 	if(prec != FVX_PREC_F64 && prec != FVX_PREC_F32)
 		throw Error(FVX_ERR_INVALID, "unknown preconditioner precision");
@@ -205,6 +348,8 @@ void pc_init(fvx_pc_s& p, const fvx_mesh_s* m, int type, int prec, int ninst)
 	}
 	if(type == FVX_PC_LINE) {
 		line_vertices(p, *m);
+	} else if(type == FVX_PC_BJACOBI_ILU0) {
+		bj_subdomains(p, *m);
 	} else {
 		p.coff.assign(p.ncol + 1, 0);
 		for(int c = 0; c < n; c++) p.coff[p.colour[c] + 1]++;
@@ -218,7 +363,7 @@ void pc_init(fvx_pc_s& p, const fvx_mesh_s* m, int type, int prec, int ninst)
 		for(int k = 0; k < 4; k++)
 			if(interior_slot(*m, c, k) && p.colour[m->esuel[(size_t)c*4 + k]] < p.colour[c])
 				low[c] |= (unsigned char)(1u << k);
-	if(n > 0) {
+	if(n > 0 && type != FVX_PC_BJACOBI_ILU0) {
 		// ILU(0) takes the lower neighbours of a row in (colour, index) order: the multicolour
 		// order's, or for the natural order the index alone
 		p.d_colour.upload(pc_level(type) ? std::vector<int>(n, 0) : p.colour);
@@ -255,21 +400,26 @@ void pc_set_sweeps(fvx_pc_s& p, int sweeps)
 
 using Op = std::function<void(const double* x, long sx, double* y, long sy, const Inst& L)>;
 
-/// Device storage of one FGMRES cycle for up to `cap` instances, per-instance scalar slots
-/// (h column, norms, solution weights) and their pinned host mirror.
+/// Device storage of one FGMRES cycle for up to `cap` instances: bases, per-instance scalar
+/// slots (h column, norms, solution weights), the least-squares state of the cycle
+/// (k_fgmres_* in linalg.hip), the cycle's instance list and its compacted running list, and
+/// pinned host words through which the device reports how many instances still run.
 struct Krylov {
 	long N = 0;
 	int m = 0, cap = 0;
-	DBuf<double> V, Z, partial, scal;
-	DBuf<int> dlist, dlist_in;      // instance lists of the caller's launches / of solve()
+	DBuf<double> V, Z, partial, scal, kst;
+	DBuf<int> dlist, dlist_in, dids, dits;  // caller's lists / running list / cycle / counts
 	double* hpin = nullptr;
 	int* ipin = nullptr;
+	int* hcnt = nullptr;                    // [m + 2] running instances after each step
+	int* itspin = nullptr;                  // [cap] iteration counts of the last cycle
+	std::vector<hipEvent_t> ev;
 	static constexpr int H0 = 0, NRM = 34, BETA = 35, XX = 36, RES = 37, Y0 = 40, NSCAL = 80;
 
 	void ensure(long n_, int m_, int cap_)
 	{
 		if(n_ == N && m_ <= m && cap_ <= cap) return;
-		release_pinned();
+		release_host();
 		N = n_;
 		m = m_;
 		cap = cap_;
@@ -277,20 +427,33 @@ struct Krylov {
 		Z.alloc((size_t)cap*m*N);
 		partial.alloc((size_t)cap*partial_stride());
 		scal.alloc((size_t)cap*NSCAL);
+		kst.alloc((size_t)cap*fgmres_state_size());
 		dlist.alloc(cap);
 		dlist_in.alloc(cap);
+		dids.alloc(cap);
+		dits.alloc(cap);
 		FVX_HIP(hipHostMalloc((void**)&hpin, (size_t)cap*NSCAL*sizeof(double)));
 		FVX_HIP(hipHostMalloc((void**)&ipin, (size_t)cap*sizeof(int)));
+		FVX_HIP(hipHostMalloc((void**)&itspin, (size_t)cap*sizeof(int)));
+		FVX_HIP(hipHostMalloc((void**)&hcnt, (size_t)(m + 2)*sizeof(int), hipHostMallocCoherent));
 		std::memset(hpin, 0, (size_t)cap*NSCAL*sizeof(double));
+		ev.assign(m + 2, nullptr);
+		for(auto& e : ev) FVX_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
 	}
-	void release_pinned()
+	void release_host()
 	{
 		if(hpin) (void)hipHostFree(hpin);
 		if(ipin) (void)hipHostFree(ipin);
+		if(itspin) (void)hipHostFree(itspin);
+		if(hcnt) (void)hipHostFree(hcnt);
+		for(auto& e : ev) if(e) (void)hipEventDestroy(e);
+		ev.clear();
 		hpin = nullptr;
 		ipin = nullptr;
+		itspin = nullptr;
+		hcnt = nullptr;
 	}
-	~Krylov() { release_pinned(); }
+	~Krylov() { release_host(); }
 
 	long sV() const { return (long)(m + 1)*N; }
 	long sZ() const { return (long)m*N; }
@@ -298,16 +461,14 @@ struct Krylov {
 
 	/// Launch description for the instances `ids`: no list for a single-instance solve,
 	/// otherwise the list uploaded to the device (the upload is waited for, so the pinned
-	/// staging buffer can be reused at once). The caller's lists and those of solve() live in
-	/// separate device buffers, so a list the caller holds stays valid across solve().
-	Inst inst(const std::vector<int>& ids, bool single, hipStream_t s, bool internal = false)
+	/// staging buffer can be reused at once).
+	Inst inst(const std::vector<int>& ids, bool single, hipStream_t s)
 	{
 		if(single) return Inst{nullptr, 1};
-		int* dst = internal ? dlist_in.p : dlist.p;
 		std::memcpy(ipin, ids.data(), ids.size()*sizeof(int));
-		FVX_HIP(hipMemcpyAsync(dst, ipin, ids.size()*sizeof(int), hipMemcpyHostToDevice, s));
+		FVX_HIP(hipMemcpyAsync(dlist.p, ipin, ids.size()*sizeof(int), hipMemcpyHostToDevice, s));
 		FVX_HIP(hipStreamSynchronize(s));
-		return Inst{dst, (int)ids.size()};
+		return Inst{dlist.p, (int)ids.size()};
 	}
 
 	/// Device scalar slots of every instance to the host (one copy, then waits).
@@ -319,37 +480,30 @@ struct Krylov {
 	}
 
 	/// One cycle of right-preconditioned FGMRES from x = 0 with classical Gram-Schmidt for the
-	/// instances `ids` (instance 0 alone when `single`). b and x are instance-major with
-	/// stride N. Returns the iteration count per instance in `its`, or -1 where a non-finite
-	/// value appeared (that instance's x is then undefined).
+	/// instances `ids`. b and x are instance-major with stride N. Returns the iteration count
+	/// per instance in `its`, or -1 where a non-finite value appeared (that instance's x is then
+	/// undefined). The least-squares update runs on the device after every Arnoldi step and
+	/// compacts the list of running instances there, so the launches of the next step skip the
+	/// stopped ones; the host only looks one step behind (has any instance still been running
+	/// after the previous step?) to decide whether to enqueue another step, so it never waits
+	/// for the step it has just enqueued. The launches take the device list also for a single
+	/// instance (the list {0}).
 	void solve(const Op& op, const Op& pc, const double* b, double* x, int mm, double rtol,
 	           const std::vector<int>& ids, bool single, std::vector<int>& its, hipStream_t s)
 	{
+		(void)single;
 		its.assign(cap, 0);
-		Inst L = inst(ids, single, s, true);
-		launch_mdot(N, b, 0, N, 1, b, N, partial.p, scal.p + BETA, NSCAL, L, s);
-		fetch(s);
-		struct St { std::vector<double> H, cs, sn, g; double beta = 0; int it = 0; };
-		std::vector<St> st(cap);
-		std::vector<int> act;
-		for(int i : ids) {
-			const double beta = std::sqrt(h(i, BETA));
-			if(!std::isfinite(beta)) { its[i] = -1; continue; }
-			if(beta == 0.0) {
-				FVX_HIP(hipMemsetAsync(x + (size_t)i*N, 0, N*sizeof(double), s));
-				continue;
-			}
-			St& t = st[i];
-			t.H.assign((size_t)(mm + 1)*mm, 0.0);
-			t.cs.assign(mm, 0.0); t.sn.assign(mm, 0.0); t.g.assign(mm + 1, 0.0);
-			t.beta = beta; t.g[0] = beta;
-			act.push_back(i);
-		}
-		if(act.empty()) { FVX_HIP(hipStreamSynchronize(s)); return; }
-		const std::vector<int> started = act;
-		if(act.size() != ids.size()) L = inst(act, single, s, true);
+		const int nids = (int)ids.size();
+		if(nids == 0) return;
+		std::memcpy(ipin, ids.data(), nids*sizeof(int));
+		FVX_HIP(hipMemcpyAsync(dids.p, ipin, nids*sizeof(int), hipMemcpyHostToDevice, s));
+		const Inst A{dids.p, nids};            // every instance of the cycle
+		const Inst L{dlist_in.p, nids};        // the running ones, compacted on the device
+		launch_mdot(N, b, 0, N, 1, b, N, partial.p, scal.p + BETA, NSCAL, A, s);
+		launch_fgmres_begin(nids, dids.p, scal.p, NSCAL, BETA, kst.p, dlist_in.p, hcnt, s);
+		FVX_HIP(hipEventRecord(ev[0], s));
 		launch_scale(N, b, N, scal.p + BETA, NSCAL, V.p, sV(), L, s);
-		for(int j = 0; j < mm && !act.empty(); j++) {
+		for(int j = 0; j < mm; j++) {
 			double* vj = V.p + (size_t)j*N;
 			double* zj = Z.p + (size_t)j*N;
 			double* w = V.p + (size_t)(j + 1)*N;
@@ -360,63 +514,20 @@ struct Krylov {
 			                 scal.p + NRM, NSCAL, L, s);
 			// normalise v_{j+1}; after an exact breakdown (norm 0) the vector is never used
 			launch_scale(N, w, sV(), scal.p + NRM, NSCAL, w, sV(), L, s);
-			fetch(s);
-			std::vector<int> next;
-			for(int i : act) {
-				St& t = st[i];
-				double* H = t.H.data();
-				for(int r = 0; r <= j; r++) H[(size_t)r*mm + j] = h(i, H0 + r);
-				const double hn = std::sqrt(h(i, NRM));
-				if(!std::isfinite(hn)) { its[i] = -1; continue; }
-				H[(size_t)(j + 1)*mm + j] = hn;
-				for(int r = 0; r < j; r++) {
-					const double a = t.cs[r]*H[(size_t)r*mm + j] + t.sn[r]*H[(size_t)(r + 1)*mm + j];
-					H[(size_t)(r + 1)*mm + j] = -t.sn[r]*H[(size_t)r*mm + j]
-					                            + t.cs[r]*H[(size_t)(r + 1)*mm + j];
-					H[(size_t)r*mm + j] = a;
-				}
-				const double a = H[(size_t)j*mm + j], c = H[(size_t)(j + 1)*mm + j];
-				const double den = std::sqrt(a*a + c*c);
-				t.cs[j] = a/den;
-				t.sn[j] = c/den;
-				H[(size_t)j*mm + j] = den;
-				H[(size_t)(j + 1)*mm + j] = 0.0;
-				t.g[j + 1] = -t.sn[j]*t.g[j];
-				t.g[j] = t.cs[j]*t.g[j];
-				t.it = j + 1;
-				if(!(std::fabs(t.g[j + 1]) <= rtol*t.beta || hn == 0.0)) next.push_back(i);
-			}
-			if(!next.empty() && next.size() != act.size()) L = inst(next, single, s, true);
-			act.swap(next);
+			launch_fgmres_rotate(j, rtol, nids, dids.p, scal.p, NSCAL, H0, NRM, kst.p,
+			                     dlist_in.p, hcnt + j + 1, s);
+			FVX_HIP(hipEventRecord(ev[j + 1], s));
+			// step j is queued; stop if nothing ran after step j - 1 (step j then found an
+			// empty list and did nothing)
+			FVX_HIP(hipEventSynchronize(ev[j]));
+			if(__atomic_load_n(hcnt + j, __ATOMIC_ACQUIRE) == 0) break;
 		}
-		// solution weights, then x = sum_k y_k Z_k for the instances with equal counts together
-		std::vector<int> done;
-		for(int i : started) {
-			if(its[i] < 0) continue;
-			St& t = st[i];
-			for(int r = t.it - 1; r >= 0; r--) {
-				double a = t.g[r];
-				for(int k = r + 1; k < t.it; k++) a -= t.H[(size_t)r*mm + k]*h(i, Y0 + k);
-				h(i, Y0 + r) = a/t.H[(size_t)r*mm + r];
-			}
-			its[i] = t.it;
-			done.push_back(i);
-		}
-		if(done.empty()) { FVX_HIP(hipStreamSynchronize(s)); return; }
-		FVX_HIP(hipMemcpyAsync(scal.p, hpin, (size_t)cap*NSCAL*sizeof(double),
-		                       hipMemcpyHostToDevice, s));
-		std::sort(done.begin(), done.end(), [&](int a, int b) {
-			return its[a] != its[b] ? its[a] < its[b] : a < b; });
-		size_t k0 = 0;
-		while(k0 < done.size()) {
-			size_t k1 = k0;
-			while(k1 < done.size() && its[done[k1]] == its[done[k0]]) k1++;
-			const std::vector<int> grp(done.begin() + k0, done.begin() + k1);
-			const Inst G = inst(grp, single, s, true);
-			launch_lincomb(N, Z.p, N, sZ(), its[grp[0]], scal.p + Y0, NSCAL, x, N, G, s);
-			k0 = k1;
-		}
+		int* dits_p = dits.p;
+		launch_fgmres_end(nids, dids.p, kst.p, scal.p, NSCAL, Y0, dits_p, s);
+		launch_lincomb(N, Z.p, N, sZ(), mm, scal.p + Y0, NSCAL, x, N, A, s, dits_p);
+		FVX_HIP(hipMemcpyAsync(itspin, dits_p, cap*sizeof(int), hipMemcpyDeviceToHost, s));
 		FVX_HIP(hipStreamSynchronize(s));
+		for(int i : ids) its[i] = itspin[i];
 	}
 };
 
@@ -546,9 +657,14 @@ void steady_core(fvx_disc_s& d, const fvx_solver_cfg& c, double* u, int ninst,
 			w.jprec = c.jac_precision;
 		}
 		const double thr = c.line_threshold > 0 ? c.line_threshold : 10.0;
+		const int sub = c.pc_subdomain_cells > 0 ? c.pc_subdomain_cells : 256;
 		if(w.pctype != c.pc || w.pcprec != c.pc_precision
-		   || (c.pc == FVX_PC_LINE && w.P.line_threshold != thr)) {
+		   || (c.pc == FVX_PC_LINE && w.P.line_threshold != thr)
+		   || (c.pc == FVX_PC_BJACOBI_ILU0 && w.P.subdomain_cells != sub)) {
 			w.P.line_threshold = thr;
+			if(sub < 16 || sub > BJ_MAX_CELLS)
+				throw Error(FVX_ERR_INVALID, "subdomain size must be in [16, 1024] cells");
+			w.P.subdomain_cells = sub;
 			pc_init(w.P, &m, c.pc, c.pc_precision, ninst);
 			w.pctype = c.pc;
 			w.pcprec = c.pc_precision;
@@ -687,7 +803,9 @@ void steady_core(fvx_disc_s& d, const fvx_solver_cfg& c, double* u, int ninst,
 		st.final_abs = is[i].resi;
 		st.first_norm = is[i].initres;
 		st.final_rel = is[i].resi/is[i].initres;
-		st.converged = (st.status == 0 && st.final_rel <= c.tol) ? 1 : 0;
+		// converged only below the step limit: a solve that reaches the tolerance exactly on
+		// step maxiter counts as not converged (src/ode/aodesolver.cpp:618-626)
+		st.converged = (st.status == 0 && st.steps < c.maxiter && st.final_rel <= c.tol) ? 1 : 0;
 		if(st.status == 0 && !st.converged) st.status = FVX_ERR_NOT_CONVERGED;
 	}
 }

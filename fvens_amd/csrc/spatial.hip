@@ -23,14 +23,6 @@ namespace fvx {
 FVX_DECL_FLUX(0) FVX_DECL_FLUX(1) FVX_DECL_FLUX(2) FVX_DECL_FLUX(3) FVX_DECL_FLUX(4)
 FVX_DECL_FLUX(5) FVX_DECL_FLUX(6)
 #undef FVX_DECL_FLUX
-#define FVX_DECL_FLUXT(ID) void launch_flux_tasks_##ID(int ncell, const double* u, \
-	const double* rgrad, const int4* tasks, const int* tptr, const unsigned char* nslot, \
-	const FaceGeom* fg, const double* centre, const double* area, const int* bcid, \
-	const GasParams* gpa, const BCTable* bta, double* res, double* dt, int getdt, \
-	const BatchArg& ba, int nact, int xcd, hipStream_t s);
-FVX_DECL_FLUXT(0) FVX_DECL_FLUXT(1) FVX_DECL_FLUXT(2) FVX_DECL_FLUXT(3) FVX_DECL_FLUXT(4)
-FVX_DECL_FLUXT(5) FVX_DECL_FLUXT(6)
-#undef FVX_DECL_FLUXT
 
 /// LIM: 0 none, 1 Venkatakrishnan, 2 Barth-Jespersen. CONS: gradients of the conserved
 /// variables (getGradients) instead of the primitive ones. gout receives the unlimited
@@ -60,6 +52,7 @@ void k_gradients(int ncell, const double* __restrict__ u, const int* __restrict_
 	// instance: state and gradients are instance-major, 4 and 8 values per cell; gpa holds
 	// the gas parameters of each instance (one entry for a single solve)
 	const long I = ba.list ? ba.list[blockIdx.y] : 0;
+	if(I < 0) return;                     // a retired slot of a compacted list
 	const GasParams& gp = gpa[I];
 	const BCTable& bt = *bta;
 	u += I*4L*ncell;
@@ -185,6 +178,7 @@ void k_weno(int ncell, const double* __restrict__ grad, const int* __restrict__ 
             double lambda, const BatchArg ba, double* __restrict__ rout)
 {
 	const long I = ba.list ? ba.list[blockIdx.y] : 0;
+	if(I < 0) return;                     // a retired slot of a compacted list
 	grad += I*8L*ncell;
 	rout += I*8L*ncell;
 	const int v = threadIdx.x & 3;
@@ -293,13 +287,6 @@ void launch_gradients(const fvx_disc_s& d, const double* u, double* grad, double
 	d.prof.stop(s);
 }
 
-bool flux_tasks_apply(const fvx_disc_s& d)
-{
-	// linear reconstruction (with or without a limiter or WENO weights), inviscid
-	const bool order2 = d.nc.order2 && d.nc.gradientscheme != FVX_GRAD_NONE;
-	return order2 && d.nc.reconstruction != FVX_RECON_VANALBADA && !d.pc.viscous;
-}
-
 void launch_residual(const fvx_disc_s& d, const double* u, double* r, int gettimesteps,
                      double* dt, hipStream_t s, const BatchArg& ba, int nact, double* grad,
                      double* rgrad)
@@ -329,21 +316,6 @@ void launch_residual(const fvx_disc_s& d, const double* u, double* r, int gettim
 		}
 	}
 	d.prof.start(PROF_FLUX, s);
-	if(d.tune.flux_tasks && flux_tasks_apply(d)) {
-		// face-task form (flux_tasks.hpp): each face evaluated once per workgroup
-		switch(d.nc.conv_numflux) {
-#define FVX_T(ID) case ID: launch_flux_tasks_##ID(m.nelem, u, rg, m.d_ftask.p, m.d_ftptr.p, \
-	m.d_nslot.p, m.d_fgeom.p, m.d_centre.p, m.d_area.p, d.d_bcid.p, \
-	ba.gp ? ba.gp : d.d_gas.p, d.d_bct.p, r, dt, gettimesteps, ba, nact, d.tune.flux_xcd, s); \
-	break;
-		FVX_T(0) FVX_T(1) FVX_T(2) FVX_T(3) FVX_T(4) FVX_T(5) FVX_T(6)
-#undef FVX_T
-		default: throw Error(FVX_ERR_INVALID, "unknown flux");
-		}
-		FVX_HIP(hipGetLastError());
-		d.prof.stop(s);
-		return;
-	}
 	switch(d.nc.conv_numflux) {
 #define FVX_C(ID) case ID: launch_flux_##ID(mode, visc, m.nelem, u, rg, ug, m.d_nbr.p, \
 	m.d_face.p, m.d_fgeom.p, m.d_centre.p, m.d_area.p, d.d_bcid.p, \

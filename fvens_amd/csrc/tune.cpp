@@ -71,7 +71,7 @@ void autotune_launch(fvx_disc_s& d, const double* u, int reps, std::vector<fvx_t
 						launch_gradients(d, u, limited ? (visc ? d.d_grad.p : nullptr) : d.d_grad.p,
 						                 limited ? d.d_rgrad.p : nullptr, 0, s); }, reps, s);
 				} else if(kernel == FVX_KERNEL_FLUX) {
-					d.tune.flux_block = block; d.tune.flux_xcd = xcd; d.tune.flux_tasks = 0;
+					d.tune.flux_block = block; d.tune.flux_xcd = xcd;
 					// the whole residual; the gradient pass is the same for every candidate
 					t = T.ms([&] { launch_residual(d, u, r.p, 1, dt.p, s); }, reps, s);
 				} else {
@@ -83,20 +83,8 @@ void autotune_launch(fvx_disc_s& d, const double* u, int reps, std::vector<fvx_t
 				if(t < best) { best = t; bb = block; bx = xcd; }
 			}
 		}
-		// the face-task form of the flux pass (one shape, both block orders) where it applies
-		int btasks = 0;
-		if(kernel == FVX_KERNEL_FLUX && flux_tasks_apply(d)) {
-			for(int xcd = 0; xcd <= 1; xcd++) {
-				d.tune.flux_tasks = 1; d.tune.flux_xcd = xcd;
-				const double t = T.ms([&] { launch_residual(d, u, r.p, 1, dt.p, s); }, reps, s);
-				rows.push_back(fvx_tune_row{FVX_KERNEL_FLUX_TASKS, FT_THREADS, xcd, t});
-				if(t < best) { best = t; bx = xcd; btasks = 1; }
-			}
-		}
 		if(kernel == FVX_KERNEL_GRADIENTS) { d.tune.grad_block = bb; d.tune.grad_xcd = bx; }
-		else if(kernel == FVX_KERNEL_FLUX) {
-			d.tune.flux_block = bb; d.tune.flux_xcd = bx; d.tune.flux_tasks = btasks;
-		}
+		else if(kernel == FVX_KERNEL_FLUX) { d.tune.flux_block = bb; d.tune.flux_xcd = bx; }
 		else { d.tune.jac_block = bb; d.tune.jac_xcd = bx; }
 	}
 	FVX_HIP(hipStreamSynchronize(s));

@@ -16,8 +16,6 @@ namespace fvx {
 constexpr int CELLS_PER_BLOCK = 64;
 constexpr int BLOCK = 256;
 constexpr int MAX_BLOCK = 512;   ///< largest workgroup the tunable cell kernels are built for
-constexpr int FT_CELLS = 128;    ///< cells per workgroup of the face-task flux kernel
-constexpr int FT_THREADS = 256;  ///< its threads (two per cell while staging)
 
 // DPP quad_perm selectors: lane^1 = [1,0,3,2], lane^2 = [2,3,0,1]
 constexpr int DPP_XOR1 = 0xB1;

@@ -252,7 +252,7 @@ class FlowFV:
         check(lib().fvx_get_primitive_gradients(self.h, _dptr(u, 4 * n), _dptr(grads, 8 * n),
                                                 _stream()))
 
-    KERNELS = {"gradients": 0, "flux": 1, "jacobian": 2, "pc": 3, "flux_tasks": 4}
+    KERNELS = {"gradients": 0, "flux": 1, "jacobian": 2, "pc": 3}
 
     def set_launch(self, kernel, block, xcd=True):
         check(lib().fvx_set_launch(self.h, self.KERNELS[kernel], int(block), int(xcd)))
@@ -364,7 +364,8 @@ PREC = {"f64": 0, "f32": 1}
 
 class TuneRow(C.Structure):
     _fields_ = [("kernel", C.c_int), ("block", C.c_int), ("xcd", C.c_int), ("ms", C.c_double)]
-PC = {"none": 0, "jacobi": 1, "sgs": 2, "ilu0": 3, "line": 4, "sgs_level": 5, "ilu0_level": 6}
+PC = {"none": 0, "jacobi": 1, "sgs": 2, "ilu0": 3, "line": 4, "sgs_level": 5, "ilu0_level": 6,
+      "bjacobi_ilu0": 7}
 
 
 class SolverCfg(C.Structure):
@@ -373,7 +374,8 @@ class SolverCfg(C.Structure):
                 ("min_relax", C.c_double), ("pc", C.c_int), ("pc_precision", C.c_int),
                 ("pc_sweeps", C.c_int), ("jac_precision", C.c_int), ("krylov_restart", C.c_int),
                 ("krylov_rtol", C.c_double), ("matrix_free", C.c_int), ("mf_eps", C.c_double),
-                ("fixed_norm", C.c_double), ("line_threshold", C.c_double)]
+                ("fixed_norm", C.c_double), ("line_threshold", C.c_double),
+                ("pc_subdomain_cells", C.c_int)]
 
 
 class SolveStats(C.Structure):
@@ -385,7 +387,7 @@ class SolveStats(C.Structure):
 def solver_cfg(cflinit=500.0, cflfin=5000.0, tol=1e-7, maxiter=170, implicit=True,
                fullupdate=False, min_relax=0.2, pc="sgs", pc_precision="f64", pc_sweeps=1,
                jac_precision="f64", krylov_restart=30, krylov_rtol=1e-1, matrix_free=False,
-               mf_eps=1e-7, fixed_norm=0.0, line_threshold=10.0):
+               mf_eps=1e-7, fixed_norm=0.0, line_threshold=10.0, pc_subdomain_cells=256):
     """Defaults: testcases/naca0012/transonic-sanity-test-muscl.ctrl main solve with the
     linear-solver options of testcases/defaults.solverc."""
     c = SolverCfg()
@@ -396,6 +398,7 @@ def solver_cfg(cflinit=500.0, cflfin=5000.0, tol=1e-7, maxiter=170, implicit=Tru
     c.krylov_restart, c.krylov_rtol = krylov_restart, krylov_rtol
     c.matrix_free, c.mf_eps, c.fixed_norm = int(matrix_free), mf_eps, fixed_norm
     c.line_threshold = line_threshold
+    c.pc_subdomain_cells = pc_subdomain_cells
     return c
 
 
@@ -435,7 +438,8 @@ class Preconditioner:
     """Device preconditioner (none / block Jacobi / multicolour SGS / multicolour BILU(0) /
     line Gauss-Seidel)."""
 
-    def __init__(self, mesh: Mesh, kind="ilu0", precision="f64", sweeps=1, line_threshold=10.0):
+    def __init__(self, mesh: Mesh, kind="ilu0", precision="f64", sweeps=1, line_threshold=10.0,
+                 subdomain_cells=None):
         self.mesh = mesh
         h = C.c_void_p()
         if kind == "line":
@@ -445,6 +449,8 @@ class Preconditioner:
             check(lib().fvx_pc_create(mesh.h, PC[kind], PREC[precision], C.byref(h)))
         self.h = h
         check(lib().fvx_pc_set_sweeps(h, int(sweeps)))
+        if subdomain_cells is not None:
+            self.set_subdomain_cells(subdomain_cells)
         self._A = None
 
     def setup(self, A: "Matrix"):
@@ -465,6 +471,17 @@ class Preconditioner:
         nc = C.c_int()
         check(lib().fvx_pc_ordering(self.h, _ptr(col, C.c_int), C.byref(nc)))
         return col, nc.value
+
+    def set_subdomain_cells(self, cells):
+        """Block Jacobi + ILU(0): re-partitions into subdomains of about `cells` cells."""
+        check(lib().fvx_pc_set_subdomain_cells(self.h, int(cells)))
+
+    def subdomains(self):
+        """Block Jacobi + ILU(0): the subdomain of every cell and the subdomain count."""
+        part = np.zeros(self.mesh.nelem, np.int32)
+        ns = C.c_int()
+        check(lib().fvx_pc_subdomains(self.h, _ptr(part, C.c_int), C.byref(ns)))
+        return part, ns.value
 
 
 def linear_solve(A: Matrix, pc: Preconditioner, b, x, restart=30, rtol=1e-1):

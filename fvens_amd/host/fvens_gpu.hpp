@@ -327,7 +327,8 @@ struct SteadySolverConfig {
 	int maxiter = 170;
 	std::string nl_update_scheme = "ROBUST_FLOW";   ///< or "FULL"
 	double min_nl_update = 0.2;
-	std::string pc_type = "sgs";                     ///< none | jacobi | sgs | ilu0 | line
+	/// none | jacobi | sgs | ilu0 | line | sgs_level | ilu0_level | bjacobi_ilu0
+	std::string pc_type = "sgs";
 	int pc_sweeps = 1;
 	int pc_precision = FVX_PREC_F64, jac_precision = FVX_PREC_F64;
 	int ksp_max_it = 30;
@@ -336,6 +337,7 @@ struct SteadySolverConfig {
 	double matrix_free_difference_step = 1e-7;
 	double fixed_norm = 0.0;
 	double line_threshold = 10.0;                    ///< anisotropy threshold of "line"
+	int pc_subdomain_cells = 256;                    ///< subdomain size of "bjacobi_ilu0"
 };
 
 struct TimingData {
@@ -362,7 +364,8 @@ public:
 		                                      {"sgs", FVX_PC_SGS}, {"ilu0", FVX_PC_ILU0},
 		                                      {"line", FVX_PC_LINE},
 		                                      {"sgs_level", FVX_PC_SGS_LEVEL},
-		                                      {"ilu0_level", FVX_PC_ILU0_LEVEL}},
+		                                      {"ilu0_level", FVX_PC_ILU0_LEVEL},
+		                                      {"bjacobi_ilu0", FVX_PC_BJACOBI_ILU0}},
 		                      "preconditioner");
 		c.pc_precision = conf_.pc_precision;
 		c.pc_sweeps = conf_.pc_sweeps;
@@ -373,6 +376,7 @@ public:
 		c.mf_eps = conf_.matrix_free_difference_step;
 		c.fixed_norm = conf_.fixed_norm;
 		c.line_threshold = conf_.line_threshold;
+		c.pc_subdomain_cells = conf_.pc_subdomain_cells;
 		fvx_solve_stats st{};
 		const fvx_status s = fvx_steady_solve(space_->handle(), &c, u, &st, stream);
 		tdata_.ode_walltime = st.walltime;

@@ -176,8 +176,17 @@ enum { FVX_PREC_F64 = 0, FVX_PREC_F32 = 1 };
  * sweeps run over wavefront levels (cells whose lower-numbered neighbours all lie in earlier
  * levels, src/mesh/ameshutils.cpp:158-196), one parallel launch per level; the result is the
  * sequential natural-order one. fvx_pc_ordering reports the levels as colours. */
+/* FVX_PC_BJACOBI_ILU0: block Jacobi over subdomains of about fvx_pc_set_subdomain_cells cells
+ * (default 256; compact, grown breadth-first over the cell graph from the lowest-numbered
+ * unassigned cell) with block ILU(0) in the natural order inside each subdomain -- the
+ * reference's -pc_type bjacobi -sub_pc_type ilu when each rank owns one subdomain
+ * (src/ode/aodesolver.cpp:483, testcases/defaults.solverc; PETSc's -pc_bjacobi_blocks). Couplings
+ * between subdomains are dropped from the factors; the sweeps option adds stationary
+ * iterations with the full matrix. One workgroup holds one subdomain and runs its forward and
+ * backward substitution over the subdomain's wavefront levels in LDS: one launch per
+ * application. fvx_pc_ordering reports the level of each cell inside its subdomain. */
 enum { FVX_PC_NONE = 0, FVX_PC_JACOBI = 1, FVX_PC_SGS = 2, FVX_PC_ILU0 = 3, FVX_PC_LINE = 4,
-       FVX_PC_SGS_LEVEL = 5, FVX_PC_ILU0_LEVEL = 6 };
+       FVX_PC_SGS_LEVEL = 5, FVX_PC_ILU0_LEVEL = 6, FVX_PC_BJACOBI_ILU0 = 7 };
 
 typedef struct fvx_matrix_s* fvx_matrix;
 typedef struct fvx_pc_s* fvx_pc;
@@ -218,6 +227,11 @@ fvx_status fvx_pc_setup(fvx_pc p, fvx_matrix A, void* stream);
 fvx_status fvx_pc_apply(fvx_pc p, const double* r, double* z, void* stream);
 /* Host copy of the colour of every cell and the number of colours of the PC's ordering. */
 fvx_status fvx_pc_ordering(fvx_pc p, int* colour, int* ncolours);
+/* FVX_PC_BJACOBI_ILU0: cells per subdomain, in [16, 1024] (default 256); re-partitions the
+ * mesh, so a set-up must follow (the role of PETSc's -pc_bjacobi_blocks, given as a size). */
+fvx_status fvx_pc_set_subdomain_cells(fvx_pc p, int cells);
+/* FVX_PC_BJACOBI_ILU0: host copy of the subdomain of every cell and the subdomain count. */
+fvx_status fvx_pc_subdomains(fvx_pc p, int* part, int* nparts);
 
 /* KSPSolve with -ksp_type fgmres (src/ode/aodesolver.cpp:483; options
  * testcases/defaults.solverc): one restart cycle of at most `restart` iterations of right-
@@ -250,6 +264,7 @@ typedef struct {
 	double mf_eps;            /* matrix-free difference step */
 	double fixed_norm;        /* > 0: normalise residuals by this instead of the first norm */
 	double line_threshold;    /* FVX_PC_LINE: anisotropy threshold of the lines (<= 0: 10) */
+	int pc_subdomain_cells;   /* FVX_PC_BJACOBI_ILU0: cells per subdomain (<= 0: 256) */
 } fvx_solver_cfg;
 
 typedef struct {
@@ -384,12 +399,9 @@ fvx_status fvx_mpnn_train_step(fvx_mpnn n, int ngraph, const float* targets, con
 
 /* ---- timing: the TimingData role of the reference (src/ode/aodesolver.hpp:46-67) ---- */
 
-/* Kernel ids for fvx_profile_read and fvx_set_launch / fvx_get_launch. FVX_KERNEL_FLUX_TASKS
- * is the face-task form of the face-flux pass (linear reconstruction, inviscid): setting a launch
- * for it (block 256) selects that form, setting one for FVX_KERNEL_FLUX selects the quad-per-cell
- * form; both give the same bits. Its time is read as FVX_KERNEL_FLUX. */
+/* Kernel ids for fvx_profile_read and fvx_set_launch / fvx_get_launch. */
 enum { FVX_KERNEL_GRADIENTS = 0, FVX_KERNEL_FLUX = 1, FVX_KERNEL_JACOBIAN = 2,
-       FVX_KERNEL_PC = 3, FVX_KERNEL_FLUX_TASKS = 4 };
+       FVX_KERNEL_PC = 3 };
 
 /* on != 0 resets the counters and starts recording HIP events around every kernel the handle
  * launches (on the launch stream); on == 0 stops recording. */

@@ -331,7 +331,8 @@ SolveStats steady_solve(const Disc& d, const SolverConfig& sc, double* u)
 	st.steps = step;
 	st.final_abs = resi;
 	st.final_rel = resi/initres;
-	st.converged = (st.status == 0 && resi/initres <= sc.tol) ? 1 : 0;
+	// converged only below the step limit (src/ode/aodesolver.cpp:618-626)
+	st.converged = (st.status == 0 && step < sc.maxiter && resi/initres <= sc.tol) ? 1 : 0;
 	if(st.status == 0 && !st.converged) st.status = 4;
 	st.walltime = std::chrono::duration<double>(std::chrono::steady_clock::now()-t0).count();
 	return st;
