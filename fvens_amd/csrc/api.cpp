@@ -6,6 +6,8 @@
 #include <cstring>
 #include <memory>
 #include <functional>
+#include <mutex>
+#include <string>
 
 namespace {
 thread_local std::string g_last_error;
@@ -28,6 +30,39 @@ void require(bool c, const char* msg)
 namespace fvx {
 /// The status conversion of this boundary for entry points defined in other source files.
 fvx_status guard(const std::function<void()>& f) { return guarded(f); }
+}
+
+namespace {
+/* Scratch of fvx_matfree_apply_noalloc: perturbed state and its residual (4 values per cell
+ * each, FVX_NOALLOC_MAX_CELLS cells) and the norm partial sums (NOALLOC_PARTIAL values). The
+ * array's name is a host-side stand-in: host code uses only the device address that
+ * hipGetSymbolAddress returns for the current device, kept per device. */
+constexpr long NOALLOC_PARTIAL = 512L*32 + 1;
+static __device__ __align__(256) double scratch[8L*FVX_NOALLOC_MAX_CELLS + NOALLOC_PARTIAL];
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[NOALLOC_MAX_DEVICES] = {};
+
+/// The current device's copy of `scratch`, looked up on the first call on that device.
+double* noalloc_scratch()
+{
+	if(fvx::partial_stride() + 1 > NOALLOC_PARTIAL)
+		throw fvx::Error(FVX_ERR_INVALID, "no-allocation scratch smaller than the partial sums");
+	int dev = 0;
+	FVX_HIP(hipGetDevice(&dev));
+	if(dev < 0 || dev >= NOALLOC_MAX_DEVICES)
+		throw fvx::Error(FVX_ERR_INVALID, "device index beyond the no-allocation table");
+	std::lock_guard<std::mutex> lock(g_scratch_mutex);
+	if(!g_scratch_ptr[dev]) {
+		void* p = nullptr;
+		const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if(e != hipSuccess || !p)
+			throw fvx::Error(FVX_ERR_HIP, std::string("hipGetSymbolAddress(scratch): ")
+			                              + hipGetErrorString(e));
+		g_scratch_ptr[dev] = static_cast<double*>(p);
+	}
+	return g_scratch_ptr[dev];
+}
 }
 
 extern "C" {
@@ -343,6 +378,25 @@ fvx_status fvx_matfree_apply(fvx_disc d, const double* u, const double* r_u, con
 		}
 		fvx::matfree_apply(*d, u, r_u, dtm, x, y, eps, d->mf_aux.p, d->mf_raux.p, d->mf_part.p,
 		                   d->mf_part.p + (size_t)fvx::partial_stride(), s);
+	});
+}
+
+fvx_status fvx_matfree_apply_noalloc(fvx_disc d, const double* u, const double* r_u,
+                                     const double* dtm, const double* x, double* y, double eps,
+                                     void* stream)
+{
+	return guarded([&] {
+		require(d && u && r_u && dtm && x && y, "null argument");
+		require(eps > 0, "difference step must be positive");
+		require(d->m->nelem <= FVX_NOALLOC_MAX_CELLS,
+		        "mesh larger than the no-allocation scratch (FVX_NOALLOC_MAX_CELLS cells)");
+		double* ptr = noalloc_scratch();
+		double* aux = ptr;
+		double* raux = ptr + 4L*FVX_NOALLOC_MAX_CELLS;
+		double* part = ptr + 8L*FVX_NOALLOC_MAX_CELLS;
+		// matfree_apply writes aux and the partial sums in full and clears raux itself
+		fvx::matfree_apply(*d, u, r_u, dtm, x, y, eps, aux, raux, part,
+		                   part + fvx::partial_stride(), static_cast<hipStream_t>(stream));
 	});
 }
 

@@ -319,6 +319,14 @@ class FlowFV:
                                       _dptr(x, 4 * n), _dptr(y, 4 * n), C.c_double(eps),
                                       _stream()))
 
+    def matfree_apply_noalloc(self, u, r_u, dtm, x, y, eps=1e-7):
+        """matfree_apply through the entry point that allocates no device memory (meshes of at
+        most FVX_NOALLOC_MAX_CELLS cells)."""
+        n = self.mesh.nelem
+        check(lib().fvx_matfree_apply_noalloc(self.h, _dptr(u, 4 * n), _dptr(r_u, 4 * n),
+                                              _dptr(dtm, n), _dptr(x, 4 * n), _dptr(y, 4 * n),
+                                              C.c_double(eps), _stream()))
+
     def steady_solve(self, u, raise_on_failure=True, **kw):
         """Runs the pseudo-time loop on the device tensor u (updated in place); returns the
         statistics as a dict. Keyword arguments override solver_cfg() defaults."""

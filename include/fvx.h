@@ -246,6 +246,19 @@ fvx_status fvx_linear_solve(fvx_matrix A, fvx_pc p, const double* b, double* x, 
 fvx_status fvx_matfree_apply(fvx_disc d, const double* u, const double* r_u, const double* dtm,
                              const double* x, double* y, double eps, void* stream);
 
+/* Largest mesh (cells) of fvx_matfree_apply_noalloc. */
+#define FVX_NOALLOC_MAX_CELLS 131072
+/* fvx_matfree_apply without any device allocation. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc). Same arguments and result bits as fvx_matfree_apply; its
+ * scratch (perturbed state, its residual, norm partial sums: 8.5 MB) is a static device array
+ * that every GPU the library runs on holds until the process ends. Meshes above
+ * FVX_NOALLOC_MAX_CELLS cells are rejected with FVX_ERR_INVALID. Every call on one GPU uses the
+ * same scratch: calls that overlap on one GPU (different streams, host threads) race. */
+fvx_status fvx_matfree_apply_noalloc(fvx_disc d, const double* u, const double* r_u,
+                                     const double* dtm, const double* x, double* y, double eps,
+                                     void* stream);
+
 /* SteadySolverConfig (src/ode/aodesolver.hpp:16-42) plus the linear-solver options. */
 typedef struct {
 	double cflinit, cflfin;   /* CFL ramp bounds */
