@@ -54,6 +54,8 @@ def parse():
     p.add_argument("--warm-params", default=os.path.join(ROOT, "fvens_amd", "data",
                                                           "warmstart_k2.npz"))
     p.add_argument("--warm-points", type=int, default=32)
+    p.add_argument("--no-warm-start", action="store_true",
+                   help="skip the warm-start A/B among the time-to-tolerance legs")
     p.add_argument("--only", choices=["residual", "solve", "batch", "mpnn"], default=None,
                    help="run one leg only (for profiling); the JSON line then lacks the others")
     p.add_argument("--no-solve", action="store_true",
@@ -427,6 +429,58 @@ def cpu_baseline(po, flags, raw, bcs, u, budget_s, budget_1t):
                 host=host_cpu())
 
 
+def time_to_tol(a):
+    """Second half of the metric: pseudo-time solves to a residual tolerance on the device,
+    with the CPU oracle beside them unless --no-cpu-baseline."""
+    bump64 = gpu_time_to_tol("bump", 1e-10, "f64")
+    bump32 = gpu_time_to_tol("bump", 1e-10, "f32")
+    air64 = gpu_time_to_tol("airfoil", 1e-8, "f64")
+    air32 = gpu_time_to_tol("airfoil", 1e-8, "f32")
+    tt = {
+        "criterion": "sqrt(sum r_energy^2 area) <= tol x the same norm at the free stream",
+        "bump_1e-10": {
+            "case": "Gaussian bump 96x32 hybrid, 4,608 cells, M0.5, Roe, Green-Gauss, "
+                    "unlimited; first-order starter then main solve",
+            "gpu_fp64": bump64, "gpu_fp32_jacobian_and_pc": bump32,
+            "fp32_speedup": round(bump64["seconds"] / bump32["seconds"], 3),
+            "warm_start": None},
+        "airfoil_1e-8": {
+            "case": "NACA 4-digit O-mesh 512x128, 98,304 cells, M0.5, alpha 1.25, HLLC, WLS, "
+                    "Venkatakrishnan K=5 (the limiter stalls the residual near 3e-9, so "
+                    "1e-8 is the airfoil target)",
+            "gpu_fp64": air64, "gpu_fp32_jacobian_and_pc": air32,
+            "fp32_speedup": round(air64["seconds"] / air32["seconds"], 3),
+            "CL_CDp_fp32_vs_fp64": [abs(air32["CL"] / air64["CL"] - 1),
+                                    abs(air32["CDp"] / air64["CDp"] - 1)]},
+    }
+    if not a.no_cpu_baseline:
+        tt["bump_1e-10"]["cpu_oracle_fp64"] = cpu_time_to_tol("bump", 1e-10)
+        tt["bump_1e-10"]["gpu_vs_cpu"] = round(
+            tt["bump_1e-10"]["cpu_oracle_fp64"]["seconds"] / bump64["seconds"], 2)
+    v64 = viscous_time_to_tol("f64")
+    v32 = viscous_time_to_tol("f32")
+    tt["viscous_naca0012_1e-10"] = {
+        "case": "the reference's laminar NACA0012 case (13,156-cell hybrid mesh, Roe, least "
+                "squares, adiabatic wall, M0.5, Re 5000), starter then main solve to 1e-10",
+        "gpu_fp64": v64, "gpu_fp32_jacobian_and_pc": v32,
+        "fp32_speedup": round(v64["seconds"] / v32["seconds"], 3),
+        "CL_CDp_CDsf_fp32_vs_fp64": [abs(v32["CL"] - v64["CL"]),
+                                     abs(v32["CDp"] / v64["CDp"] - 1),
+                                     abs(v32["CDsf"] / v64["CDsf"] - 1)]}
+    if not a.no_cpu_baseline:
+        vo = viscous_time_to_tol("f64", oracle=True)
+        tt["viscous_naca0012_1e-10"]["cpu_oracle_fp64"] = vo
+        tt["viscous_naca0012_1e-10"]["gpu_vs_cpu"] = round(vo["seconds"] / v64["seconds"], 2)
+    if a.no_warm_start:
+        tt["warm_start_airfoil"] = {"skipped": "--no-warm-start"}
+    elif os.path.exists(a.warm_params):
+        tt["warm_start_airfoil"] = warm_start_ab(a.warm_params, a.warm_points, 4,
+                                                 a.batch_maxiter)
+    else:
+        tt["warm_start_airfoil"] = {"skipped": "no trained parameters at " + a.warm_params}
+    return tt
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -447,8 +501,9 @@ def main():
         print(json.dumps({"mpnn": mpnn_timing(8, min(a.steps, 50))}), flush=True)
         return
     if a.only == "solve":
-        a.no_batch = True
-    elif a.only == "residual":
+        print(json.dumps({"time_to_tol": time_to_tol(a)}), flush=True)
+        return
+    if a.only == "residual":
         a.no_batch = a.no_solve = True
     raw, bcs, pcfg, ncfg, u_host = build_case(a.order)
     mesh = fvx.Mesh(raw)
@@ -528,51 +583,7 @@ def main():
     if not a.no_solve:
         # second half of the metric: time to 1e-10 of the free-stream residual norm
         del disc, mesh, u, r, dt
-        bump64 = gpu_time_to_tol("bump", 1e-10, "f64")
-        bump32 = gpu_time_to_tol("bump", 1e-10, "f32")
-        air64 = gpu_time_to_tol("airfoil", 1e-8, "f64")
-        air32 = gpu_time_to_tol("airfoil", 1e-8, "f32")
-        tt = {
-            "criterion": "sqrt(sum r_energy^2 area) <= tol x the same norm at the free stream",
-            "bump_1e-10": {
-                "case": "Gaussian bump 96x32 hybrid, 4,608 cells, M0.5, Roe, Green-Gauss, "
-                        "unlimited; first-order starter then main solve",
-                "gpu_fp64": bump64, "gpu_fp32_jacobian_and_pc": bump32,
-                "fp32_speedup": round(bump64["seconds"] / bump32["seconds"], 3),
-                "warm_start": None},
-            "airfoil_1e-8": {
-                "case": "NACA 4-digit O-mesh 512x128, 98,304 cells, M0.5, alpha 1.25, HLLC, WLS, "
-                        "Venkatakrishnan K=5 (the limiter stalls the residual near 3e-9, so "
-                        "1e-8 is the airfoil target)",
-                "gpu_fp64": air64, "gpu_fp32_jacobian_and_pc": air32,
-                "fp32_speedup": round(air64["seconds"] / air32["seconds"], 3),
-                "CL_CDp_fp32_vs_fp64": [abs(air32["CL"] / air64["CL"] - 1),
-                                        abs(air32["CDp"] / air64["CDp"] - 1)]},
-        }
-        if not a.no_cpu_baseline:
-            tt["bump_1e-10"]["cpu_oracle_fp64"] = cpu_time_to_tol("bump", 1e-10)
-            tt["bump_1e-10"]["gpu_vs_cpu"] = round(
-                tt["bump_1e-10"]["cpu_oracle_fp64"]["seconds"] / bump64["seconds"], 2)
-        v64 = viscous_time_to_tol("f64")
-        v32 = viscous_time_to_tol("f32")
-        tt["viscous_naca0012_1e-10"] = {
-            "case": "the reference's laminar NACA0012 case (13,156-cell hybrid mesh, Roe, least "
-                    "squares, adiabatic wall, M0.5, Re 5000), starter then main solve to 1e-10",
-            "gpu_fp64": v64, "gpu_fp32_jacobian_and_pc": v32,
-            "fp32_speedup": round(v64["seconds"] / v32["seconds"], 3),
-            "CL_CDp_CDsf_fp32_vs_fp64": [abs(v32["CL"] - v64["CL"]),
-                                         abs(v32["CDp"] / v64["CDp"] - 1),
-                                         abs(v32["CDsf"] / v64["CDsf"] - 1)]}
-        if not a.no_cpu_baseline:
-            vo = viscous_time_to_tol("f64", oracle=True)
-            tt["viscous_naca0012_1e-10"]["cpu_oracle_fp64"] = vo
-            tt["viscous_naca0012_1e-10"]["gpu_vs_cpu"] = round(vo["seconds"] / v64["seconds"], 2)
-        if os.path.exists(a.warm_params):
-            tt["warm_start_airfoil"] = warm_start_ab(a.warm_params, a.warm_points, 4,
-                                                     a.batch_maxiter)
-        else:
-            tt["warm_start_airfoil"] = {"skipped": "no trained parameters at " + a.warm_params}
-        out["time_to_tol"] = tt
+        out["time_to_tol"] = time_to_tol(a)
     if not a.no_batch:
         out["batch"] = batch_sweep(a.batch_instances, a.batch_maxiter)
         out["mpnn"] = mpnn_timing(8, 20)
