@@ -903,41 +903,45 @@ void launch_pc_setup(fvx_pc_s& p, const fvx_matrix_s& A, const Inst& L, hipStrea
 	}
 }
 
-/// One forward and one backward pass of symmetric Gauss-Seidel over the colours; the first
-/// forward pass (from z = 0) reads lower colours only.
+/// `sweeps` forward + backward passes of symmetric Gauss-Seidel over the colours from z = 0;
+/// the first forward pass reads lower colours only. A colour pass reads only the other colours,
+/// so a pass that directly follows a pass of the same colour (the highest colour where the
+/// forward sweep turns back, the lowest where the next sweep starts) would recompute the same
+/// values from the same data, bit for bit: it is skipped, which leaves 2 sweeps (colours - 1) + 1
+/// launches instead of 2 sweeps colours.
 template <class VTF>
 static void sgs_passes(const fvx_pc_s& p, const VTF* fac, const double* r, long sr, double* z,
-                       long sz, bool first, const Inst& L, hipStream_t s)
+                       long sz, int sweeps, const Inst& L, hipStream_t s)
 {
 	const int* nbr = p.m->d_nbr.p;
 	const long sa = mstride(p.m);
-	for(int c = 0; c < p.ncol; c++) {
+	int last = -1;                          // colour of the previous launched pass
+	auto pass = [&](int c, bool all) {
 		const int cnt = p.coff[c+1] - p.coff[c];
-		if(cnt == 0) continue;
+		if(cnt == 0 || c == last) return;
 		const int* list = p.d_order.p + p.coff[c];
 		const dim3 g(quad_blocks(cnt), L.nact);
-		if(first)
-			hipLaunchKernelGGL((k_sgs_colour<VTF, false>), g, dim3(BLOCK), 0, s, cnt, list,
-			                   L.list, fac, sa, nbr, p.d_lowmask.p, r, sr, z, sz);
-		else
+		if(all)
 			hipLaunchKernelGGL((k_sgs_colour<VTF, true>), g, dim3(BLOCK), 0, s, cnt, list, L.list,
 			                   fac, sa, nbr, p.d_lowmask.p, r, sr, z, sz);
+		else
+			hipLaunchKernelGGL((k_sgs_colour<VTF, false>), g, dim3(BLOCK), 0, s, cnt, list,
+			                   L.list, fac, sa, nbr, p.d_lowmask.p, r, sr, z, sz);
 		FVX_HIP(hipGetLastError());
-	}
-	for(int c = p.ncol - 1; c >= 0; c--) {
-		const int cnt = p.coff[c+1] - p.coff[c];
-		if(cnt == 0) continue;
-		hipLaunchKernelGGL((k_sgs_colour<VTF, true>), dim3(quad_blocks(cnt), L.nact), dim3(BLOCK),
-		                   0, s, cnt, p.d_order.p + p.coff[c], L.list, fac, sa, nbr,
-		                   p.d_lowmask.p, r, sr, z, sz);
-		FVX_HIP(hipGetLastError());
+		last = c;
+	};
+	for(int k = 0; k < sweeps; k++) {
+		for(int c = 0; c < p.ncol; c++) pass(c, k > 0);
+		for(int c = p.ncol - 1; c >= 0; c--) pass(c, true);
 	}
 }
 
-/// One forward and one backward pass of line Gauss-Seidel over the colours of the vertices.
+/// `sweeps` forward + backward passes of line Gauss-Seidel over the colours of the vertices
+/// from z = 0, skipping a colour pass that directly follows a pass of the same colour (it would
+/// recompute the same values, as for point Gauss-Seidel).
 template <class VTF>
 static void line_passes(const fvx_pc_s& p, const VTF* fac, const double* r, long sr, double* z,
-                        long sz, bool first, const Inst& L, hipStream_t s)
+                        long sz, int sweeps, const Inst& L, hipStream_t s)
 {
 	const int* nbr = p.m->d_nbr.p;
 	const long sa = mstride(p.m);
@@ -970,8 +974,16 @@ static void line_passes(const fvx_pc_s& p, const VTF* fac, const double* r, long
 			FVX_HIP(hipGetLastError());
 		}
 	};
-	for(int c = 0; c < p.ncol; c++) colour(c, !first);
-	for(int c = p.ncol - 1; c >= 0; c--) colour(c, true);
+	int last = -1;                          // colour of the previous pass
+	auto pass = [&](int c, bool all) {
+		if(c == last || p.coff[c+1] == p.coff[c]) return;
+		colour(c, all);
+		last = c;
+	};
+	for(int k = 0; k < sweeps; k++) {
+		for(int c = 0; c < p.ncol; c++) pass(c, k > 0);
+		for(int c = p.ncol - 1; c >= 0; c--) pass(c, true);
+	}
 }
 
 /// z = M^-1 r for Jacobi or ILU(0) (one application).
@@ -1020,11 +1032,11 @@ static void pc_apply_t(const fvx_pc_s& p, const VTF* fac, const double* r, long 
                        long sz, const Inst& L, hipStream_t s)
 {
 	if(pc_sgs(p.type)) {
-		for(int k = 0; k < p.sweeps; k++) sgs_passes(p, fac, r, sr, z, sz, k == 0, L, s);
+		sgs_passes(p, fac, r, sr, z, sz, p.sweeps, L, s);
 		return;
 	}
 	if(p.type == FVX_PC_LINE) {
-		for(int k = 0; k < p.sweeps; k++) line_passes(p, fac, r, sr, z, sz, k == 0, L, s);
+		line_passes(p, fac, r, sr, z, sz, p.sweeps, L, s);
 		return;
 	}
 	pc_once(p, fac, r, sr, z, sz, L, s);

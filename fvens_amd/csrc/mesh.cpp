@@ -201,6 +201,9 @@ void build_mesh(fvx_mesh_s& m, const fvx_mesh_desc& d)
 			const int o = m.esuel[(size_t)c*4+j];
 			nbr[(size_t)c*4+j] = o < m.nelem ? o : -1 - (o - m.nelem);
 			fidx[(size_t)c*4+j] = m.elemface[(size_t)c*4+j];
+			// the kernels find boundary face ib's record at index ib
+			if(o >= m.nelem && fidx[(size_t)c*4+j] != o - m.nelem)
+				throw Error(FVX_ERR_INVALID, "boundary faces are not numbered first");
 		}
 	std::vector<FaceGeom> fg(m.naface);
 	for(int i = 0; i < m.naface; i++)

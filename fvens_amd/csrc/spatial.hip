@@ -42,7 +42,7 @@ FVX_DECL_FLUX(5) FVX_DECL_FLUX(6)
 template <int LIM, bool CONS>
 __global__ __launch_bounds__(MAX_BLOCK)
 void k_gradients(int ncell, const double* __restrict__ u, const int* __restrict__ nbr,
-                 const int* __restrict__ face, const FaceGeom* __restrict__ fgeom,
+                 const FaceGeom* __restrict__ fgeom,
                  const double* __restrict__ soff, const double* __restrict__ gcoef,
                  const double* __restrict__ eps2, const int* __restrict__ bcid,
                  const GasParams* __restrict__ gpa, const BCTable* __restrict__ bta,
@@ -74,9 +74,7 @@ void k_gradients(int ncell, const double* __restrict__ u, const int* __restrict_
 	else prim_lean(gp, ui, wi);
 	if(valid && p == 0) for(int k = 0; k < 4; k++) sw[k][lc] = wi[k];
 	const int2 nb2 = reinterpret_cast<const int2*>(nbr)[2*cc + p];
-	const int2 fc2 = reinterpret_cast<const int2*>(face)[2*cc + p];
 	const int nbs[2] = { valid ? nb2.x : SLOT_UNUSED, valid ? nb2.y : SLOT_UNUSED };
-	const int fcs[2] = { fc2.x, fc2.y };
 	const double2* cf = reinterpret_cast<const double2*>(gcoef) + 4*cc + 2*p;
 	// neighbours outside the workgroup's range: both gathers in flight together (any other
 	// slot loads the cell itself as a placeholder)
@@ -108,7 +106,8 @@ void k_gradients(int ncell, const double* __restrict__ u, const int* __restrict_
 			for(int v = 0; v < 4; v++) wj[v] = sw[v][nb - c0];
 		} else {
 			if(nb < 0) {
-				const FaceGeom G = fgeom[fcs[k]];
+				// boundary face ib is face ib (boundary faces are numbered first)
+				const FaceGeom G = fgeom[-1 - nb];
 				const double n[2] = { G.nx, G.ny };
 				ghost_state(gp, bt, bcid[-1 - nb], ui, n, uj[k]);
 			}
@@ -274,7 +273,7 @@ void launch_gradients(const fvx_disc_s& d, const double* u, double* grad, double
 	const int lim = conserved ? 0 : (d.nc.reconstruction == FVX_RECON_VENKATAKRISHNAN ? 1 :
 	                                 d.nc.reconstruction == FVX_RECON_BARTHJESPERSEN ? 2 : 0);
 #define FVX_GL(L, C) hipLaunchKernelGGL((k_gradients<L, C>), dim3(nblk, nact), dim3(block), 0, \
-		s, m.nelem, u, m.d_nbr.p, m.d_face.p, m.d_fgeom.p, m.d_soff.p, d.d_gcoef.p, \
+		s, m.nelem, u, m.d_nbr.p, m.d_fgeom.p, m.d_soff.p, d.d_gcoef.p, \
 		d.d_eps2.p, d.d_bcid.p, ba.gp ? ba.gp : d.d_gas.p, d.d_bct.p, ba, grad, rgrad, \
 		d.tune.grad_xcd)
 	d.prof.start(PROF_GRADIENTS, s);

@@ -32,12 +32,14 @@ from .points import operating_points
 AIRFOIL_BCS = [(2, "slipwall"), (4, "farfield")]
 
 
-def discretisations(gm):
+def discretisations(gm, limiter_param=5.0):
+    """First-order starter and second-order operator of the sweep (Roe, weighted least squares,
+    Venkatakrishnan with parameter K = limiter_param)."""
     d1 = fvx.FlowFV(gm, fvx.physics_cfg(Minf=0.5, bcs=AIRFOIL_BCS),
                     fvx.numerics_cfg(flux="ROE", gradient="NONE", order2=False))
     d2 = fvx.FlowFV(gm, fvx.physics_cfg(Minf=0.5, bcs=AIRFOIL_BCS),
                     fvx.numerics_cfg(flux="ROE", gradient="LEASTSQUARES",
-                                     recon="VENKATAKRISHNAN", limiter_param=5.0))
+                                     recon="VENKATAKRISHNAN", limiter_param=limiter_param))
     return d1, d2
 
 

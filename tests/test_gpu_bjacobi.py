@@ -141,3 +141,39 @@ def test_subdomain_size_is_validated(torch_cuda):
     Q = fvx.Preconditioner(gm, "ilu0", "f64")
     with pytest.raises(fvx.FvxError):
         Q.subdomains()
+
+
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_bump_solve_with_subdomain_ilu_matches_the_oracle(torch_cuda, prec):
+    # the Gaussian bump (Roe, Green-Gauss, unlimited) solved to 1e-10 of its free-stream norm
+    # with block Jacobi + ILU(0) over 256-cell subdomains: the converged lift, drag and entropy
+    # error equal the oracle's natural-order ILU(0) solve
+    import torch
+    import bench
+    raw, phys, n1, n2, ramps = bench._solve_case("bump")
+    m = po.Mesh(raw)
+    o1 = po.Disc(m, po.make_config(**phys, **n1))
+    o2 = po.Disc(m, po.make_config(**phys, **n2))
+    u0 = np.tile(o1.freestream(), (raw.nelem, 1))
+    r, _ = o2.residual(u0)
+    norm0 = float(np.sqrt((r[:, 3] ** 2 * m.area).sum()))
+    uo, _ = o1.steady_solve(u0.copy(), cflinit=ramps[0][0], cflfin=ramps[0][1], tol=1e-1,
+                            maxiter=300, fullupdate=0, min_relax=0.2, pc="ilu0")
+    uo, so = o2.steady_solve(uo, cflinit=ramps[1][0], cflfin=ramps[1][1], tol=1e-10,
+                             maxiter=3000, fullupdate=0, min_relax=0.2, pc="ilu0",
+                             fixed_norm=norm0)
+    assert so["converged"]
+    fo = o2.functionals(uo, 2)
+    gm = fvx.Mesh(raw)
+    g1 = fvx.FlowFV(gm, fvx.physics_cfg(**phys), fvx.numerics_cfg(**n1))
+    g2 = fvx.FlowFV(gm, fvx.physics_cfg(**phys), fvx.numerics_cfg(**n2))
+    u = torch.tensor(u0, device="cuda")
+    kw = dict(min_relax=0.2, pc="bjacobi_ilu0", pc_sweeps=1, pc_subdomain_cells=256,
+              pc_precision=prec, jac_precision=prec, raise_on_failure=False)
+    g1.steady_solve(u, cflinit=ramps[0][0], cflfin=ramps[0][1], tol=1e-1, maxiter=300, **kw)
+    st = g2.steady_solve(u, cflinit=ramps[1][0], cflfin=ramps[1][1], tol=1e-10, maxiter=3000,
+                         fixed_norm=norm0, **kw)
+    assert st["converged"], st
+    f = g2.functionals(u, 2)
+    assert abs(f["entropy"] / fo["entropy"] - 1) <= 1e-8, (f, fo)
+    assert abs(f["CDp"] / fo["CDp"] - 1) <= 1e-6 and abs(f["CL"] / fo["CL"] - 1) <= 1e-6
