@@ -195,6 +195,7 @@ struct fvx_pc_s {
 	std::vector<int> colour, order, coff;
 	fvx::DBuf<int> d_colour, d_order;
 	fvx::DBuf<unsigned char> d_lowmask;
+	fvx::DBuf<int4> d_crec;                // SGS: per position {cell, lower mask, slots 0-3}
 	// line Gauss-Seidel: vertices (lines, then the other cells) listed by colour in order /
 	// coff, their cells, and per cell the slots of its line neighbours
 	double line_threshold = 10.0;

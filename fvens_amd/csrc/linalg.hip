@@ -257,6 +257,40 @@ void k_sgs_colour(int cnt, const int* __restrict__ list, const int* __restrict__
 	if(valid) z[4*i + q] = dot4(a, tv);
 }
 
+/// The same colour pass with the colour's cells given as records {cell, lower-colour mask,
+/// neighbour slots 0-3} (two int4 per position): the identical arithmetic, one dependent load
+/// less (the cell's neighbour slots come with its index instead of after it).
+template <class VTF, bool ALL>
+__global__ __launch_bounds__(BLOCK)
+void k_sgs_rec(int cnt, const int4* __restrict__ rec, const int* __restrict__ ilist,
+               const VTF* __restrict__ fac, long sa, const double* __restrict__ r, long sr,
+               double* __restrict__ z, long sz)
+{
+	const long I = inst_of(ilist);
+	if(I < 0) return;                     // a retired slot of a compacted list
+	fac += I*sa; r += I*sr; z += I*sz;
+	const int q = threadIdx.x & 3;
+	const int idx = xcd_block()*CELLS_PER_BLOCK + (threadIdx.x >> 2);
+	const bool valid = idx < cnt;
+	const int e = valid ? idx : 0;
+	const int4 ra = rec[2*e], rb = rec[2*e + 1];
+	const int i = ra.x, lm = ra.y;
+	const int nbs[4] = { ra.z, ra.w, rb.x, rb.y };
+	const VTF* blk = fac + (size_t)i*80 + 4*q;
+	double t = r[4*i + q];
+	double a[4], zv[4];
+	for(int k = 0; k < 4; k++) {
+		if(nbs[k] < 0 || !(ALL || ((lm >> k) & 1))) continue;
+		load_row<VTF>(blk + 16*(1 + k), a);
+		load4(z + 4*nbs[k], zv);
+		t -= dot4(a, zv);
+	}
+	const double tv[4] = { quad_bcast<0>(t), quad_bcast<1>(t), quad_bcast<2>(t),
+	                       quad_bcast<3>(t) };
+	load_row<VTF>(blk, a);
+	if(valid) z[4*i + q] = dot4(a, tv);
+}
+
 /// Forward substitution of block ILU(0) for one colour: y_i = r_i - sum_{lower k} L_ik y_k.
 template <class VTF>
 __global__ __launch_bounds__(BLOCK)
@@ -913,20 +947,19 @@ template <class VTF>
 static void sgs_passes(const fvx_pc_s& p, const VTF* fac, const double* r, long sr, double* z,
                        long sz, int sweeps, const Inst& L, hipStream_t s)
 {
-	const int* nbr = p.m->d_nbr.p;
 	const long sa = mstride(p.m);
 	int last = -1;                          // colour of the previous launched pass
 	auto pass = [&](int c, bool all) {
 		const int cnt = p.coff[c+1] - p.coff[c];
 		if(cnt == 0 || c == last) return;
-		const int* list = p.d_order.p + p.coff[c];
+		const int4* rec = p.d_crec.p + 2L*p.coff[c];
 		const dim3 g(quad_blocks(cnt), L.nact);
 		if(all)
-			hipLaunchKernelGGL((k_sgs_colour<VTF, true>), g, dim3(BLOCK), 0, s, cnt, list, L.list,
-			                   fac, sa, nbr, p.d_lowmask.p, r, sr, z, sz);
+			hipLaunchKernelGGL((k_sgs_rec<VTF, true>), g, dim3(BLOCK), 0, s, cnt, rec, L.list,
+			                   fac, sa, r, sr, z, sz);
 		else
-			hipLaunchKernelGGL((k_sgs_colour<VTF, false>), g, dim3(BLOCK), 0, s, cnt, list,
-			                   L.list, fac, sa, nbr, p.d_lowmask.p, r, sr, z, sz);
+			hipLaunchKernelGGL((k_sgs_rec<VTF, false>), g, dim3(BLOCK), 0, s, cnt, rec, L.list,
+			                   fac, sa, r, sr, z, sz);
 		FVX_HIP(hipGetLastError());
 		last = c;
 	};

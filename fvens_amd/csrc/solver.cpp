@@ -369,6 +369,22 @@ void pc_init(fvx_pc_s& p, const fvx_mesh_s* m, int type, int prec, int ninst)
 		p.d_colour.upload(pc_level(type) ? std::vector<int>(n, 0) : p.colour);
 		p.d_order.upload(p.order);
 		p.d_lowmask.upload(low);
+		if(pc_sgs(type)) {
+			// per position of the colour order: {cell, lower-colour mask, neighbour slots 0-3},
+			// the slots as the mesh's (neighbour, -1-ib on a boundary, SLOT_UNUSED)
+			std::vector<int4> rec((size_t)2*n);
+			for(int e = 0; e < n; e++) {
+				const int c = p.order[e];
+				int nb[4];
+				for(int k = 0; k < 4; k++) {
+					const int o = m->esuel[(size_t)c*4 + k];
+					nb[k] = k >= m->nnode[c] ? SLOT_UNUSED : o < n ? o : -1 - (o - n);
+				}
+				rec[2*e] = make_int4(c, low[c], nb[0], nb[1]);
+				rec[2*e + 1] = make_int4(nb[2], nb[3], 0, 0);
+			}
+			p.d_crec.upload(rec);
+		}
 	}
 	p.one.upload(std::vector<double>(1, 1.0));
 	if(type != FVX_PC_NONE) {
