@@ -1,6 +1,12 @@
 """Trains the learned warm start of the inviscid airfoil sweep and measures what it gains.
 
     python -m fvens_amd.train_warmstart --ntrain 2000 --ntest 200 --out DIR
+    python -m fvens_amd.train_warmstart --case bump --ntrain 256 --ntest 64 --tol 1e-10 \
+        --maxiter 3000 --steps 1500 --out DIR
+
+The second form trains the same network on the Gaussian bump channel (4,608 cells, Roe,
+Green-Gauss, unlimited) over inflow Mach U[0.3, 0.6] and angle U[-2, 2] deg, whose solves reach
+1e-10 of the free-stream residual norm.
 
 1. Training set: `ntrain` operating points (Mach ~ U[0.3, 0.75], AoA ~ U[-4, 6] deg from
    std::mt19937_64 seed 2000) on the 98,304-cell NACA 4-digit O-mesh (Roe, weighted least
@@ -30,6 +36,51 @@ from . import warmstart as ws
 from .points import operating_points
 
 AIRFOIL_BCS = [(2, "slipwall"), (4, "farfield")]
+BUMP_BCS = [(2, "slipwall"), (3, "inflowoutflow"), (4, "inflowoutflow")]
+# operating-point ranges (Mach, angle of the free stream in degrees) of each case's family
+RANGES = {"airfoil": ((0.3, 0.75), (-4.0, 6.0)), "bump": ((0.3, 0.6), (-2.0, 2.0))}
+
+
+def case_mesh(case):
+    """The case's mesh (RCM order for the airfoil, the generator's order for the bump)."""
+    if case == "bump":
+        return meshgen.mesh_k1()
+    raw = meshgen.mesh_k2()
+    return raw.permuted(fvx.rcm_permutation(raw))
+
+
+def case_points(case, seed, count):
+    """Seeded operating points of the case's family (std::mt19937_64)."""
+    mach, aoa = RANGES[case]
+    return operating_points(seed, count, mach=mach, aoa_deg=aoa)
+
+
+def case_setup(case, gm, sweeps, maxiter, tol):
+    """(starter discretisation, main discretisation, FP32 and FP64 preconditioner settings,
+    starter settings, main settings) of the case. The bump family is the Gaussian channel of
+    tests/inv-gaussianbump (Roe, Green-Gauss, unlimited; starter CFL 25 -> 500, main
+    250 -> 2000 from its base.ctrl) over a range of inflow Mach numbers and angles."""
+    if case != "bump":
+        d1, d2 = discretisations(gm)
+        return (d1, d2) + solver_settings(sweeps, maxiter, tol)
+    d1 = fvx.FlowFV(gm, fvx.physics_cfg(Minf=0.5, bcs=BUMP_BCS),
+                    fvx.numerics_cfg(flux="ROE", gradient="NONE", order2=False))
+    d2 = fvx.FlowFV(gm, fvx.physics_cfg(Minf=0.5, bcs=BUMP_BCS),
+                    fvx.numerics_cfg(flux="ROE", gradient="GREENGAUSS", recon="NONE"))
+    pc32 = dict(pc="sgs", pc_sweeps=sweeps, pc_precision="f32", jac_precision="f32",
+                min_relax=0.2)
+    pc64 = dict(pc32, pc_precision="f64", jac_precision="f64")
+    starter = dict(cflinit=25.0, cflfin=500.0, tol=1e-1, maxiter=300)
+    # matrix-free Newton-Krylov with a high CFL ceiling: 15 main steps to 1e-10 where defect
+    # correction at the control file's ceiling (2000) needs 40, in the same time
+    main = dict(cflinit=250.0, cflfin=1e5, tol=tol, maxiter=maxiter, matrix_free=True)
+    return d1, d2, pc32, pc64, starter, main
+
+
+def case_warm(case):
+    """(starting CFL of the warm main solve, first-order pre-steps) of the case: the airfoil's
+    predictions need a few low-CFL first-order steps at the trailing edge, the bump's do not."""
+    return (1e4, 0) if case == "bump" else (1000.0, WARM_PRE_STEPS)
 
 
 def discretisations(gm, limiter_param=5.0):
@@ -73,19 +124,22 @@ def warm_settings(main, cflinit):
 WARM_PRE_STEPS, WARM_PRE_CFL = 5, 50.0
 
 
-def warm_solve(net, d1, d2, mach, aoa, pc32, main, norms, warm_cfl=1000.0):
-    """Warm-started solve of the points: prediction, first-order smoothing, main solve.
-    Returns the states, the main-solve statistics and the time of each phase."""
+def warm_solve(net, d1, d2, mach, aoa, pc32, main, norms, warm_cfl=1000.0,
+               pre_steps=WARM_PRE_STEPS):
+    """Warm-started solve of the points: prediction, first-order smoothing (pre_steps steps),
+    main solve. Returns the states, the main-solve statistics and the time of each phase."""
     import torch
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     u = ws.warm_states(net, mach, aoa)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    b1 = fvx.Batch(d1, mach, aoa)
-    b1.set_norms(norms)
-    pre = b1.steady_solve(u, cflinit=WARM_PRE_CFL, cflfin=WARM_PRE_CFL, tol=1e-300,
-                          maxiter=WARM_PRE_STEPS, **pc32)
+    pre = [dict(steps=0)]*len(mach)
+    if pre_steps > 0:
+        b1 = fvx.Batch(d1, mach, aoa)
+        b1.set_norms(norms)
+        pre = b1.steady_solve(u, cflinit=WARM_PRE_CFL, cflfin=WARM_PRE_CFL, tol=1e-300,
+                              maxiter=pre_steps, **pc32)
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     _, o = ws.solve_batch(d2, mach, aoa, u=u, main_cfg=dict(warm_settings(main, warm_cfl), **pc32),
@@ -143,7 +197,7 @@ def generate_targets_warm(net, d1, d2, mach, aoa, pc32, starter, main, chunk, lo
 
 
 def ab_test(net, d1, d2, mach, aoa, pc32, pc64, starter, main, log, warm_cfl=1000.0,
-            chunk=64):
+            chunk=64, pre_steps=WARM_PRE_STEPS):
     """Batched A/B over the points (in batches of `chunk`) plus per-point statistics."""
     pts = []
     tot = dict(t_cold=0.0, t_warm=0.0, t_inference=0.0, t_pre=0.0, t_fallback=0.0, fallbacks=0)
@@ -155,7 +209,7 @@ def ab_test(net, d1, d2, mach, aoa, pc32, pc64, starter, main, log, warm_cfl=100
                                  main_cfg=dict(main, **pc64), norms=norms)
         log(f"A/B points {c0}..{c0 + len(m) - 1}: plain {cold['t_starter'] + cold['t_main']:.2f} s")
         start_ratio = b.residual_norms(ws.warm_states(net, m, a)) / norms
-        _, warm = warm_solve(net, d1, d2, m, a, pc32, main, norms, warm_cfl)
+        _, warm = warm_solve(net, d1, d2, m, a, pc32, main, norms, warm_cfl, pre_steps)
         failed = [i for i, s in enumerate(warm["main"]) if s["status"] != 0]
         fb = None
         if failed:
@@ -192,7 +246,8 @@ def ab_test(net, d1, d2, mach, aoa, pc32, pc64, starter, main, log, warm_cfl=100
     return out
 
 
-def single_ab(net, d1, d2, mach, aoa, pc32, pc64, starter, main, log, warm_cfl=1000.0):
+def single_ab(net, d1, d2, mach, aoa, pc32, pc64, starter, main, log, warm_cfl=1000.0,
+              pre_steps=WARM_PRE_STEPS):
     import torch
     rows = []
     for i in range(len(mach)):
@@ -201,7 +256,7 @@ def single_ab(net, d1, d2, mach, aoa, pc32, pc64, starter, main, log, warm_cfl=1
         norm = b.residual_norms(ws.freestream_states(b))
         _, c = ws.solve_batch(d2, m1, a1, starter=d1, starter_cfg=dict(starter, **pc64),
                               main_cfg=dict(main, **pc64), norms=norm)
-        _, w = warm_solve(net, d1, d2, m1, a1, pc32, main, norm, warm_cfl)
+        _, w = warm_solve(net, d1, d2, m1, a1, pc32, main, norm, warm_cfl, pre_steps)
         r = dict(mach=float(m1[0]), aoa=float(a1[0]), t_cold=c["t_starter"] + c["t_main"],
                  t_warm=w["t_inference"] + w["t_pre"] + w["t_main"],
                  t_inference=w["t_inference"],
@@ -220,6 +275,9 @@ def single_ab(net, d1, d2, mach, aoa, pc32, pc64, starter, main, log, warm_cfl=1
 
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--case", choices=["airfoil", "bump"], default="airfoil",
+                    help="operating-point family: the 98,304-cell airfoil or the 4,608-cell "
+                         "Gaussian bump channel")
     ap.add_argument("--ntrain", type=int, default=2000)
     ap.add_argument("--ntest", type=int, default=200)
     ap.add_argument("--single", type=int, default=8)
@@ -231,7 +289,8 @@ def main():
     ap.add_argument("--sweeps", type=int, default=16)
     ap.add_argument("--chunk", type=int, default=64)
     ap.add_argument("--params", default=None, help="start from these parameters (npz)")
-    ap.add_argument("--warm-cfl", type=float, default=1000.0)
+    ap.add_argument("--warm-cfl", type=float, default=None,
+                    help="starting CFL of the warm main solve (default: the case's)")
     ap.add_argument("--skip", type=int, default=0,
                     help="use training draws skip .. skip+ntrain-1 of the seed-2000 sequence")
     ap.add_argument("--warm-labels", action="store_true",
@@ -247,15 +306,14 @@ def main():
         print(s, flush=True)
 
     os.makedirs(a.out, exist_ok=True)
-    raw = meshgen.mesh_k2()
-    raw = raw.permuted(fvx.rcm_permutation(raw))
+    raw = case_mesh(a.case)
     gm = fvx.Mesh(raw)
-    d1, d2 = discretisations(gm)
-    pc32, pc64, starter, main_cfg = solver_settings(a.sweeps, a.maxiter, a.tol)
+    d1, d2, pc32, pc64, starter, main_cfg = case_setup(a.case, gm, a.sweeps, a.maxiter, a.tol)
+    name = "warmstart_k2" if a.case == "airfoil" else "warmstart_bump"
     net = fvx.MPNN(gm, 2, a.graphs)
     res = dict(args=vars(a), cells=raw.nelem, nparams=net.nparams())
     if a.ntrain > 0:
-        mtr, atr = operating_points(2000, a.skip + a.ntrain)
+        mtr, atr = case_points(a.case, 2000, a.skip + a.ntrain)
         mtr, atr = mtr[a.skip:], atr[a.skip:]
         t0 = time.perf_counter()
         label = dict(main_cfg, tol=a.label_tol)
@@ -281,16 +339,18 @@ def main():
         res["t_train"] = time.perf_counter() - t0
         del T
         torch.cuda.empty_cache()
-        np.savez(os.path.join(a.out, "warmstart_k2.npz"), params=net.get_params())
+        np.savez(os.path.join(a.out, name + ".npz"), params=net.get_params())
     else:
         net.set_params(np.load(a.params)["params"])
     if a.ntest > 0:
-        mte, ate = operating_points(200, a.ntest)
+        mte, ate = case_points(a.case, 200, a.ntest)
+        wcfl, pre = case_warm(a.case)
+        wcfl = a.warm_cfl if a.warm_cfl is not None else wcfl
         res["ab"] = ab_test(net, d1, d2, mte, ate, pc32, pc64, starter, main_cfg, log,
-                            a.warm_cfl)
+                            wcfl, pre_steps=pre)
         res["single"] = single_ab(net, d1, d2, mte[:a.single], ate[:a.single], pc32, pc64,
-                                  starter, main_cfg, log, a.warm_cfl)
-    with open(os.path.join(a.out, "warmstart_k2.json"), "w") as fh:
+                                  starter, main_cfg, log, wcfl, pre_steps=pre)
+    with open(os.path.join(a.out, name + ".json"), "w") as fh:
         json.dump(res, fh, default=float)
     log("done")
 
