@@ -2,9 +2,12 @@
 
 Workload: inviscid transonic airfoil operator (HLLC, weighted least squares, Venkatakrishnan
 K=5, M0.8, alpha 1.25 deg) on the refined 2048x512 NACA 4-digit (t=0.12) O-mesh with the outer
-256 rings split into triangles: 1,572,864 cells, RCM-ordered. A step is one full
-FlowFV::compute_residual (gradients + limiter pass, face-flux / gather pass with local time
-steps) over a synthetic, seeded perturbed free-stream state resident in HBM.
+256 rings split into triangles: 1,572,864 cells, ordered along a Hilbert curve of the cell
+centres (--order rcm: the reference's RCM order; per-cell results agree to rounding).
+A step is one full FlowFV::compute_residual (gradients + limiter pass, face-flux / gather pass
+with local time steps) over a synthetic, seeded perturbed free-stream state resident in HBM; the
+library picks the flux pass's form (wave-task form on this order). The same workload in the
+other ordering is timed after the timed region ("other_order").
 
 Prints one JSON line. It also times pseudo-time solves to a residual tolerance (the second
 half of the metric, under "time_to_tol": the bump, the airfoil, the reference's laminar case and
@@ -45,8 +48,13 @@ def parse():
     p.add_argument("--cpu-seconds-1t", type=float, default=8.0,
                    help="bounded one-thread CPU-baseline sample (seconds)")
     p.add_argument("--no-cpu-baseline", action="store_true")
-    p.add_argument("--order", choices=["rcm", "hilbert"], default="rcm",
+    p.add_argument("--order", choices=["rcm", "hilbert"], default="hilbert",
                    help="cell ordering of the residual workload")
+    p.add_argument("--flux-form", choices=["auto", "quad", "wave"], default="auto",
+                   help="flux pass: the library's choice, a quad of lanes per cell, or face "
+                        "tasks per wave64 cell group")
+    p.add_argument("--no-alt-order", action="store_true",
+                   help="skip the residual timing in the other cell ordering")
     p.add_argument("--no-batch", action="store_true",
                    help="skip the batched operating-point sweep")
     p.add_argument("--batch-instances", type=int, default=64)
@@ -84,6 +92,55 @@ def calibrate_copy(steps):
     dt = (time.perf_counter() - t0) / steps
     print(json.dumps({"copy_bytes_read": n * 8, "copy_bytes_written": n * 8,
                       "copy_GBs": round(2 * n * 8 / dt / 1e9, 1), "launches": steps + 3}))
+
+
+def residual_leg(raw, pcfg, ncfg, u_host, flux_form, steps, warmup):
+    """Residual evaluations of one cell ordering: `steps` timed back to back between two
+    device synchronisations, then the median of per-evaluation event times (>= 50) and the
+    library's per-kernel event times, each in a loop of its own."""
+    import torch
+    from fvens_amd import fvx
+    mesh = fvx.Mesh(raw)
+    disc = fvx.FlowFV(mesh, pcfg, ncfg)
+    if flux_form == "wave":
+        disc.set_launch("flux_wave", 64, True)
+    elif flux_form == "quad":
+        disc.set_launch("flux", 256, True)
+    form = "wave" if disc.get_launch("flux_wave")[0] == 64 else "quad"
+    u = torch.tensor(u_host, device="cuda")
+    r = torch.zeros_like(u)
+    dt = torch.zeros(raw.nelem, dtype=torch.float64, device="cuda")
+    for _ in range(warmup):
+        disc.compute_residual(u, r, True, dt)
+    # the timed region: exactly `steps` residual evaluations, nothing else recorded
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        disc.compute_residual(u, r, True, dt)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    # median over >= 50 evaluations, each bracketed by events on the launch stream
+    nmed = max(50, steps)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(nmed)]
+    for e0, e1 in evs:
+        e0.record()
+        disc.compute_residual(u, r, True, dt)
+        e1.record()
+    torch.cuda.synchronize()
+    med_ms = statistics.median(e0.elapsed_time(e1) for e0, e1 in evs)
+    # per-kernel device times from the library's own event pairs, in a loop of their own
+    disc.profile_enable(True)
+    for _ in range(nmed):
+        disc.compute_residual(u, r, True, dt)
+    flux_ms, flux_n = disc.profile_read("flux")
+    grad_ms, grad_n = disc.profile_read("gradients")
+    disc.profile_enable(False)
+    assert torch.isfinite(r).all().item() and torch.isfinite(dt).all().item()
+    return {"form": form, "faces": int(mesh.naface), "nmed": nmed,
+            "ms_per_step": (t1 - t0) * 1e3 / steps,
+            "value": raw.nelem * steps / (t1 - t0) / 1e6, "median_ms": med_ms,
+            "flux_s": flux_ms / flux_n / 1e3, "grad_s": grad_ms / grad_n / 1e3}
 
 
 def build_case(order="rcm"):
@@ -520,49 +577,18 @@ def main():
     if a.only == "residual":
         a.no_batch = a.no_solve = True
     raw, bcs, pcfg, ncfg, u_host = build_case(a.order)
-    mesh = fvx.Mesh(raw)
-    disc = fvx.FlowFV(mesh, pcfg, ncfg)
-    u = torch.tensor(u_host, device="cuda")
-    r = torch.zeros_like(u)
-    dt = torch.zeros(raw.nelem, dtype=torch.float64, device="cuda")
-    for _ in range(a.warmup):
-        disc.compute_residual(u, r, True, dt)
-    # the timed region: exactly `steps` residual evaluations, nothing else recorded
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(a.steps):
-        disc.compute_residual(u, r, True, dt)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    # median over >= 50 evaluations, each bracketed by events on the launch stream
-    nmed = max(50, a.steps)
-    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
-           for _ in range(nmed)]
-    for e0, e1 in evs:
-        e0.record()
-        disc.compute_residual(u, r, True, dt)
-        e1.record()
-    torch.cuda.synchronize()
-    med_ms = statistics.median(e0.elapsed_time(e1) for e0, e1 in evs)
-    # per-kernel device times from the library's own event pairs, in a loop of their own
-    disc.profile_enable(True)
-    for _ in range(nmed):
-        disc.compute_residual(u, r, True, dt)
-    flux_ms, flux_n = disc.profile_read("flux")
-    grad_ms, grad_n = disc.profile_read("gradients")
-    disc.profile_enable(False)
-    assert torch.isfinite(r).all().item() and torch.isfinite(dt).all().item()
-    ms_per_step = (t1 - t0) * 1e3 / a.steps
-    value = raw.nelem * a.steps / (t1 - t0) / 1e6
-    flux_avg_s = flux_ms / flux_n / 1e3
-    grad_avg_s = grad_ms / grad_n / 1e3
+    res = residual_leg(raw, pcfg, ncfg, u_host, a.flux_form, a.steps, a.warmup)
+    ms_per_step, value, med_ms = res["ms_per_step"], res["value"], res["median_ms"]
+    flux_avg_s, grad_avg_s = res["flux_s"], res["grad_s"]
     achieved = FLUX_PASS_BYTES_PER_CELL * raw.nelem / flux_avg_s / 1e9
+    form = res["form"]
     traffic = None
     tf = os.path.join(ROOT, "profiles", "k_flux_traffic.json")
     if os.path.exists(tf):
         with open(tf) as fh:
             tj = json.load(fh)
-        if tj.get("order", "rcm") == a.order and tj.get("cells") == raw.nelem:
+        if (tj.get("order") == a.order and tj.get("cells") == raw.nelem
+                and tj.get("form") == form):
             traffic = tj.get("hbm_bytes_per_launch")
     out = {
         "metric": "Mcell-residuals/s; time to 1e-10 residual: warm-start+FP32 PC vs plain FP64",
@@ -573,22 +599,39 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic",
         "config": {"workload": "inviscid transonic airfoil residual, NACA 4-digit t=0.12 O-mesh "
-                               f"2048x512 (outer 256 rings split), {a.order.upper()} order",
-                   "cells": raw.nelem, "faces": int(mesh.naface), "flux": "HLLC",
+                               f"2048x512 (outer 256 rings split), {a.order.upper()} order, "
+                               + ("wave-task" if form == "wave" else "quad-per-cell")
+                               + " flux pass",
+                   "cells": raw.nelem, "faces": res["faces"], "flux": "HLLC",
                    "gradient": "weighted least squares", "limiter": "Venkatakrishnan K=5",
                    "mach": 0.8, "aoa_deg": 1.25, "state": "seeded perturbed free stream",
                    "parallelism": "single GPU"},
         "median_ms_per_evaluation": round(med_ms, 5),
         "median_value": round(raw.nelem / med_ms / 1e3, 3),
         "kernels_ms": {"k_gradients": round(grad_avg_s * 1e3, 5),
-                       "k_flux": round(flux_avg_s * 1e3, 5), "evaluations": nmed},
-        "roofline": {"bound": "hbm", "kernel": "k_flux", "achieved": round(achieved, 1),
+                       "k_flux": round(flux_avg_s * 1e3, 5), "evaluations": res["nmed"]},
+        "roofline": {"bound": "hbm", "kernel": "k_flux_wave" if form == "wave" else "k_flux",
+                     "achieved": round(achieved, 1),
                      "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes_per_cell": FLUX_PASS_BYTES_PER_CELL},
         "residual_roofline_frac": round(RESIDUAL_BYTES_PER_CELL * raw.nelem
                                         / ((flux_avg_s + grad_avg_s) * 1e9) / HBM_PEAK_GBS, 4),
     }
+    if not a.no_alt_order:
+        # the same workload in the other cell ordering (RCM is the reference's), same flux
+        # form choice, outside the timed region above
+        alt = "rcm" if a.order == "hilbert" else "hilbert"
+        raw2, _, pcfg2, ncfg2, u2 = build_case(alt)
+        r2 = residual_leg(raw2, pcfg2, ncfg2, u2, a.flux_form, a.steps, a.warmup)
+        out["other_order"] = {
+            "order": alt, "flux_form": r2["form"], "value": round(r2["value"], 3),
+            "median_value": round(raw2.nelem / r2["median_ms"] / 1e3, 3),
+            "kernels_ms": {"k_gradients": round(r2["grad_s"] * 1e3, 5),
+                           "k_flux": round(r2["flux_s"] * 1e3, 5)},
+            "k_flux_frac": round(FLUX_PASS_BYTES_PER_CELL * raw2.nelem / r2["flux_s"] / 1e9
+                                 / HBM_PEAK_GBS, 4)}
+        del raw2, u2, r2
     po = None
     if not a.no_cpu_baseline:
         po, flags = native_oracle()
@@ -596,7 +639,6 @@ def main():
                                            a.cpu_seconds_1t)
     if not a.no_solve:
         # second half of the metric: time to 1e-10 of the free-stream residual norm
-        del disc, mesh, u, r, dt
         out["time_to_tol"] = time_to_tol(a)
     if not a.no_batch:
         out["batch"] = batch_sweep(a.batch_instances, a.batch_maxiter)

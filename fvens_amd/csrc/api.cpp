@@ -591,7 +591,13 @@ fvx_status fvx_set_launch(fvx_disc d, int kernel, int block, int xcd)
 		        "block must be 64, 128, 256 or 512");
 		require(xcd == 0 || xcd == 1, "xcd must be 0 or 1");
 		if(kernel == FVX_KERNEL_GRADIENTS) { d->tune.grad_block = block; d->tune.grad_xcd = xcd; }
-		else if(kernel == FVX_KERNEL_FLUX) { d->tune.flux_block = block; d->tune.flux_xcd = xcd; }
+		else if(kernel == FVX_KERNEL_FLUX) {
+			d->tune.flux_block = block; d->tune.flux_xcd = xcd; d->tune.flux_wave = 0;
+		}
+		else if(kernel == FVX_KERNEL_FLUX_WAVE) {
+			require(block == 64, "the wave-task flux kernel runs one wave64 per workgroup");
+			d->tune.flux_xcd = xcd; d->tune.flux_wave = 1;
+		}
 		else if(kernel == FVX_KERNEL_JACOBIAN) {
 			require(block <= 256, "the Jacobian kernel takes at most 256 threads per workgroup");
 			d->tune.jac_block = block; d->tune.jac_xcd = xcd;
@@ -606,6 +612,9 @@ fvx_status fvx_get_launch(fvx_disc d, int kernel, int* block, int* xcd)
 		require(d && block && xcd, "null argument");
 		if(kernel == FVX_KERNEL_GRADIENTS) { *block = d->tune.grad_block; *xcd = d->tune.grad_xcd; }
 		else if(kernel == FVX_KERNEL_FLUX) { *block = d->tune.flux_block; *xcd = d->tune.flux_xcd; }
+		else if(kernel == FVX_KERNEL_FLUX_WAVE) {
+			*block = d->tune.flux_wave ? 64 : 0; *xcd = d->tune.flux_xcd;
+		}
 		else if(kernel == FVX_KERNEL_JACOBIAN) { *block = d->tune.jac_block; *xcd = d->tune.jac_xcd; }
 		else require(false, "kernel has no tunable launch shape");
 	});

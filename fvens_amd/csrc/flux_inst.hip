@@ -2,6 +2,7 @@
  * by the build, one object per flux so the instantiations compile in parallel) and exposes a
  * launcher selecting the reconstruction mode and the viscous variant. */
 #include "flux_kernel.hpp"
+#include "flux_wave.hpp"
 
 #ifndef FVX_FLUX_ID
 #error "FVX_FLUX_ID must be defined"
@@ -37,6 +38,26 @@ void FVX_CAT(launch_flux_, FVX_FLUX_ID)(int mode, bool visc, int ncell, const do
 		else FVX_LAUNCH(2, true);
 	}
 #undef FVX_LAUNCH
+}
+
+void FVX_CAT(launch_flux_wave_, FVX_FLUX_ID)(int mode, int ncell, int ngroup, const double* u,
+                                             const double* rgrad, const int* tasks,
+                                             const int2* groups, const unsigned* slots,
+                                             const FaceGeom* fg, const double* centre,
+                                             const double* area, const int* bcid,
+                                             const GasParams* gpa, const BCTable* bta,
+                                             double* res, double* dt, int getdt,
+                                             const BatchArg& ba, int nact, int xcd,
+                                             hipStream_t s)
+{
+	if(mode == 0)
+		hipLaunchKernelGGL((k_flux_wave<FVX_FLUX_ID, 0>), dim3(ngroup, nact), dim3(64), 0, s, ncell,
+		                   u, rgrad, tasks, groups, slots, fg, centre, area, bcid, gpa, bta, ba,
+		                   res, dt, getdt, xcd);
+	else
+		hipLaunchKernelGGL((k_flux_wave<FVX_FLUX_ID, 1>), dim3(ngroup, nact), dim3(64), 0, s, ncell,
+		                   u, rgrad, tasks, groups, slots, fg, centre, area, bcid, gpa, bta, ba,
+		                   res, dt, getdt, xcd);
 }
 
 }

@@ -24,9 +24,11 @@ struct Error : std::runtime_error {
 constexpr int SLOT_UNUSED = -0x40000000;
 
 /// Face geometry record: unit normal pointing from the left (lower) cell to the right cell,
-/// length and midpoint (48 bytes, three 16-byte loads).
+/// length, midpoint, and the two cells: left, and right or -1-ib for boundary face ib (48 bytes,
+/// three 16-byte loads).
 struct alignas(16) FaceGeom {
-	double nx, ny, len, mx, my, pad;
+	double nx, ny, len, mx, my;
+	int left, right;
 };
 
 /// Instance addressing of a launch. A batched launch covers gridDim.y instances whose ids
@@ -92,6 +94,13 @@ struct fvx_mesh_s {
 	fvx::DBuf<double> d_centre;            // [nelem][2]
 	fvx::DBuf<double> d_area;              // [nelem]
 	fvx::DBuf<int> d_bhost, d_btag;        // [nbface] host cell and marker of boundary faces
+	// wave-task form of the flux pass (flux_wave.hpp): per group of consecutive cells whose
+	// faces make at most 64 tasks, the tasks, the group's cells and each cell's slot codes
+	int nwgroup = 0;
+	fvx::DBuf<int> d_wtask;                // [ngroup][64] face of each task, -1 none
+	fvx::DBuf<int2> d_wgroup;              // [ngroup] {first cell, cells}
+	fvx::DBuf<unsigned> d_wslot;           // [ngroup][64] per cell: byte k = 2 task + side of
+	                                       // slot k, 0xff unused
 };
 
 namespace fvx {
@@ -124,6 +133,7 @@ namespace fvx {
 struct Tuning {
 	int flux_block = 256, grad_block = 256, jac_block = 256;
 	int flux_xcd = 1, grad_xcd = 1, jac_xcd = 1;
+	int flux_wave = 0;     ///< 1: wave-task form of the flux pass where it applies
 };
 }
 
@@ -233,6 +243,7 @@ void find_lines(const fvx_mesh_desc& d, double threshold, std::vector<std::vecto
                 std::vector<int>& celline);
 void line_order(const fvx_mesh_desc& d, double threshold, int graph_ordering, int* perm);
 void hilbert_order(const fvx_mesh_desc& d, int* perm);
+void build_wave_tasks(fvx_mesh_s& m);
 void build_disc(fvx_disc_s& d);
 GasParams make_gas(const fvx_physics_cfg& p);
 
@@ -241,6 +252,8 @@ GasParams make_gas(const fvx_physics_cfg& p);
 void launch_gradients(const fvx_disc_s& d, const double* u, double* grad, double* rgrad,
                       int conserved, hipStream_t s, const BatchArg& ba = BatchArg(),
                       int nact = 1);
+/// Whether the wave-task form of the flux pass covers this discretisation's options.
+bool flux_wave_apply(const fvx_disc_s& d);
 void launch_residual(const fvx_disc_s& d, const double* u, double* r, int gettimesteps,
                      double* dt, hipStream_t s, const BatchArg& ba = BatchArg(), int nact = 1,
                      double* grad = nullptr, double* rgrad = nullptr);

@@ -23,6 +23,14 @@ namespace fvx {
 FVX_DECL_FLUX(0) FVX_DECL_FLUX(1) FVX_DECL_FLUX(2) FVX_DECL_FLUX(3) FVX_DECL_FLUX(4)
 FVX_DECL_FLUX(5) FVX_DECL_FLUX(6)
 #undef FVX_DECL_FLUX
+#define FVX_DECL_FLUXW(ID) void launch_flux_wave_##ID(int mode, int ncell, int ngroup, \
+	const double* u, const double* rgrad, const int* tasks, const int2* groups, \
+	const unsigned* slots, const FaceGeom* fg, const double* centre, const double* area, \
+	const int* bcid, const GasParams* gpa, const BCTable* bta, double* res, double* dt, \
+	int getdt, const BatchArg& ba, int nact, int xcd, hipStream_t s);
+FVX_DECL_FLUXW(0) FVX_DECL_FLUXW(1) FVX_DECL_FLUXW(2) FVX_DECL_FLUXW(3) FVX_DECL_FLUXW(4)
+FVX_DECL_FLUXW(5) FVX_DECL_FLUXW(6)
+#undef FVX_DECL_FLUXW
 
 /// LIM: 0 none, 1 Venkatakrishnan, 2 Barth-Jespersen. CONS: gradients of the conserved
 /// variables (getGradients) instead of the primitive ones. gout receives the unlimited
@@ -286,6 +294,13 @@ void launch_gradients(const fvx_disc_s& d, const double* u, double* grad, double
 	d.prof.stop(s);
 }
 
+bool flux_wave_apply(const fvx_disc_s& d)
+{
+	// inviscid, first-order or linear reconstruction (unlimited, limited or WENO-weighted)
+	const bool order2 = d.nc.order2 && d.nc.gradientscheme != FVX_GRAD_NONE;
+	return !d.pc.viscous && !(order2 && d.nc.reconstruction == FVX_RECON_VANALBADA);
+}
+
 void launch_residual(const fvx_disc_s& d, const double* u, double* r, int gettimesteps,
                      double* dt, hipStream_t s, const BatchArg& ba, int nact, double* grad,
                      double* rgrad)
@@ -315,6 +330,21 @@ void launch_residual(const fvx_disc_s& d, const double* u, double* r, int gettim
 		}
 	}
 	d.prof.start(PROF_FLUX, s);
+	if(d.tune.flux_wave && flux_wave_apply(d)) {
+		// wave-task form (flux_wave.hpp): each face evaluated once per wave64 cell group
+		switch(d.nc.conv_numflux) {
+#define FVX_W(ID) case ID: launch_flux_wave_##ID(mode, m.nelem, m.nwgroup, u, rg, m.d_wtask.p, \
+	m.d_wgroup.p, m.d_wslot.p, m.d_fgeom.p, m.d_centre.p, m.d_area.p, d.d_bcid.p, \
+	ba.gp ? ba.gp : d.d_gas.p, d.d_bct.p, r, dt, gettimesteps, ba, nact, d.tune.flux_xcd, s); \
+	break;
+		FVX_W(0) FVX_W(1) FVX_W(2) FVX_W(3) FVX_W(4) FVX_W(5) FVX_W(6)
+#undef FVX_W
+		default: throw Error(FVX_ERR_INVALID, "unknown flux");
+		}
+		FVX_HIP(hipGetLastError());
+		d.prof.stop(s);
+		return;
+	}
 	switch(d.nc.conv_numflux) {
 #define FVX_C(ID) case ID: launch_flux_##ID(mode, visc, m.nelem, u, rg, ug, m.d_nbr.p, \
 	m.d_face.p, m.d_fgeom.p, m.d_centre.p, m.d_area.p, d.d_bcid.p, \

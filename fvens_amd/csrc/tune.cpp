@@ -3,9 +3,11 @@
  *
  * Launch shapes: every candidate (threads per workgroup x XCD-aware block order) of the
  * gradient, face-flux and Jacobian kernels is timed with HIP events on the caller's stream at
- * the caller's state, and the fastest is kept in the discretisation. The kernels compute each
- * cell independently of how cells are grouped into workgroups, so results are bitwise the same
- * for every candidate.
+ * the caller's state, and the fastest is kept in the discretisation; for the face-flux pass the
+ * wave-task form (both block orders) competes with the quad-per-cell shapes where it applies.
+ * The kernels compute each cell independently of how cells are grouped into workgroups, and
+ * the two flux forms evaluate the same face fluxes and sum them in the same order, so results
+ * are bitwise the same for every candidate.
  *
  * Preconditioner schedule: for the linear system of one pseudo-time step at the given state
  * and CFL number, each preconditioner type and sweep count is timed over set-up plus one
@@ -55,13 +57,16 @@ void autotune_launch(fvx_disc_s& d, const double* u, int reps, std::vector<fvx_t
 	const int rec = d.nc.reconstruction;
 	const bool limited = rec == FVX_RECON_VENKATAKRISHNAN || rec == FVX_RECON_BARTHJESPERSEN;
 	const bool visc = d.pc.viscous != 0;
+	const bool wave = flux_wave_apply(d);
 	for(int kernel = FVX_KERNEL_GRADIENTS; kernel <= FVX_KERNEL_JACOBIAN; kernel++) {
 		if(kernel == FVX_KERNEL_GRADIENTS && !order2) continue;
 		if(kernel == FVX_KERNEL_JACOBIAN && !jac) continue;
 		double best = 1e300;
-		int bb = 256, bx = 1;
-		for(int bi = 0; bi < 4; bi++) {
-			const int block = blocks[bi];
+		int bb = 256, bx = 1, bw = 0;
+		// the face-flux pass: the quad-per-cell form's shapes, then the wave-task form
+		for(int bi = 0; bi < (kernel == FVX_KERNEL_FLUX && wave ? 5 : 4); bi++) {
+			const bool wform = bi == 4;
+			const int block = wform ? 64 : blocks[bi];
 			if(kernel == FVX_KERNEL_JACOBIAN && block > 256) continue;   // its launch bound
 			for(int xcd = 0; xcd <= 1; xcd++) {
 				double t;
@@ -71,7 +76,8 @@ void autotune_launch(fvx_disc_s& d, const double* u, int reps, std::vector<fvx_t
 						launch_gradients(d, u, limited ? (visc ? d.d_grad.p : nullptr) : d.d_grad.p,
 						                 limited ? d.d_rgrad.p : nullptr, 0, s); }, reps, s);
 				} else if(kernel == FVX_KERNEL_FLUX) {
-					d.tune.flux_block = block; d.tune.flux_xcd = xcd;
+					if(!wform) d.tune.flux_block = block;
+					d.tune.flux_xcd = xcd; d.tune.flux_wave = wform;
 					// the whole residual; the gradient pass is the same for every candidate
 					t = T.ms([&] { launch_residual(d, u, r.p, 1, dt.p, s); }, reps, s);
 				} else {
@@ -79,12 +85,18 @@ void autotune_launch(fvx_disc_s& d, const double* u, int reps, std::vector<fvx_t
 					t = T.ms([&] { launch_jacobian(d, u, A.raw(), A.prec, nullptr, 0.0, nullptr, s); },
 					         reps, s);
 				}
-				rows.push_back(fvx_tune_row{kernel, block, xcd, t});
-				if(t < best) { best = t; bb = block; bx = xcd; }
+				rows.push_back(fvx_tune_row{wform ? (int)FVX_KERNEL_FLUX_WAVE : kernel, block, xcd, t});
+				if(t < best) { best = t; bb = block; bx = xcd; bw = wform; }
 			}
 		}
 		if(kernel == FVX_KERNEL_GRADIENTS) { d.tune.grad_block = bb; d.tune.grad_xcd = bx; }
-		else if(kernel == FVX_KERNEL_FLUX) { d.tune.flux_block = bb; d.tune.flux_xcd = bx; }
+		else if(kernel == FVX_KERNEL_FLUX) {
+			// the quad form keeps its fastest shape even when the wave form wins
+			double bq = 1e300;
+			for(const fvx_tune_row& w : rows)
+				if(w.kernel == FVX_KERNEL_FLUX && w.ms < bq) { bq = w.ms; d.tune.flux_block = w.block; }
+			d.tune.flux_xcd = bx; d.tune.flux_wave = bw;
+		}
 		else { d.tune.jac_block = bb; d.tune.jac_xcd = bx; }
 	}
 	FVX_HIP(hipStreamSynchronize(s));

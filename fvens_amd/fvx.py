@@ -252,7 +252,7 @@ class FlowFV:
         check(lib().fvx_get_primitive_gradients(self.h, _dptr(u, 4 * n), _dptr(grads, 8 * n),
                                                 _stream()))
 
-    KERNELS = {"gradients": 0, "flux": 1, "jacobian": 2, "pc": 3}
+    KERNELS = {"gradients": 0, "flux": 1, "jacobian": 2, "pc": 3, "flux_wave": 4}
 
     def set_launch(self, kernel, block, xcd=True):
         check(lib().fvx_set_launch(self.h, self.KERNELS[kernel], int(block), int(xcd)))

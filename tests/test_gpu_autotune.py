@@ -45,17 +45,30 @@ def test_results_do_not_depend_on_launch_shape(torch_cuda):
             r, dt, J = _eval(raw, gm, d, u)
             assert torch.equal(r, r0) and torch.equal(dt, dt0), (block, xcd)
             assert np.array_equal(J, J0), (block, xcd)
+    # the wave-task form of the flux pass
+    for xcd in (False, True):
+        d.set_launch("flux_wave", 64, xcd)
+        r, dt, J = _eval(raw, gm, d, u)
+        assert torch.equal(r, r0) and torch.equal(dt, dt0), ("wave", xcd)
 
 
 def test_autotuner_keeps_the_fastest_shape(torch_cuda):
     raw, gm, d, u = _case()
     table = d.autotune(u, reps=5)
-    assert len(table) == 8 + 8 + 6
-    for k in ("gradients", "flux", "jacobian"):
-        rows = [t for t in table if t[0] == k]
-        best = min(rows, key=lambda t: t[3])
+    # 8 shapes each for the gradient and quad-form flux kernels, the wave-task flux form in
+    # both block orders, 6 Jacobian shapes
+    assert len(table) == 8 + 8 + 2 + 6
+    assert all(t[3] > 0 for t in table)
+    for k in ("gradients", "jacobian"):
+        best = min((t for t in table if t[0] == k), key=lambda t: t[3])
         assert d.get_launch(k) == (best[1], best[2])
-        assert all(t[3] > 0 for t in rows)
+    quad = min((t for t in table if t[0] == "flux"), key=lambda t: t[3])
+    best = min((t for t in table if t[0] in ("flux", "flux_wave")), key=lambda t: t[3])
+    assert d.get_launch("flux")[0] == quad[1]
+    if best[0] == "flux_wave":
+        assert d.get_launch("flux_wave") == (64, best[2])
+    else:
+        assert d.get_launch("flux") == (best[1], best[2]) and d.get_launch("flux_wave")[0] == 0
 
 
 def test_preconditioner_tuner_returns_a_schedule_that_converges(torch_cuda):
