@@ -161,7 +161,7 @@ BUMP_BCS = [(2, "slipwall"), (3, "inflowoutflow"), (4, "inflowoutflow")]
 AIRFOIL_BCS = [(2, "slipwall"), (4, "farfield")]
 
 
-def _solve_case(name):
+def _solve_case(name, order="rcm"):
     """Mesh, physics and (first-order, second-order) numerics of a time-to-tolerance case."""
     from fvens_amd import fvx, meshgen
     if name == "bump":
@@ -178,7 +178,7 @@ def _solve_case(name):
         n2 = dict(flux="HLLC", gradient="LEASTSQUARES", recon="VENKATAKRISHNAN",
                   limiter_param=5.0)
         ramps = ((1000.0, 1000.0), (1000.0, 10000.0))
-    raw = raw.permuted(fvx.rcm_permutation(raw))
+    raw = raw.permuted(fvx.rcm_permutation(raw) if order == "rcm" else fvx.hilbert_permutation(raw))
     return raw, phys, n1, n2, ramps
 
 
@@ -187,13 +187,13 @@ PC_NAMES = {"sgs": "multicolour SGS", "ilu0": "multicolour block ILU(0)",
             "bjacobi_ilu0": "block Jacobi + ILU(0)"}
 
 
-def gpu_time_to_tol(name, tol, precision, sweeps=16, pc="sgs", subdomain=256):
+def gpu_time_to_tol(name, tol, precision, sweeps=16, pc="sgs", subdomain=256, order="rcm"):
     """Pseudo-time solve from free stream (first-order starter, then the main solve) until
     sqrt(sum r_E^2 area) <= tol x the same norm at the free-stream state."""
     import numpy as np
     import torch
     from fvens_amd import fvx
-    raw, phys, n1, n2, ramps = _solve_case(name)
+    raw, phys, n1, n2, ramps = _solve_case(name, order)
     mesh = fvx.Mesh(raw)
     phys_cfg = fvx.physics_cfg(**phys)
     d1 = fvx.FlowFV(mesh, phys_cfg, fvx.numerics_cfg(**n1))
