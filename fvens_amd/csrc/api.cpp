@@ -438,6 +438,7 @@ fvx_status fvx_batch_create(fvx_disc d, int ninst, const double* mach, const dou
 		const size_t n = (size_t)d->m->nelem;
 		b->d_grad.alloc(8*n*ninst);
 		b->d_rgrad.alloc(8*n*ninst);
+		b->d_ughost.alloc(4*(size_t)std::max(d->m->nbface, 1)*ninst);
 		b->d_scratch.alloc((size_t)fvx::functional_blocks()*5);
 		*out = b.release();
 	});
@@ -467,7 +468,7 @@ fvx_status fvx_batch_compute_residual(fvx_batch b, const double* u, double* r, i
 		ba.list = b->d_all.p;
 		ba.gp = b->d_gas.p;
 		fvx::launch_residual(*b->d, u, r, gettimesteps, dt, static_cast<hipStream_t>(stream), ba,
-		                     b->ninst, b->d_grad.p, b->d_rgrad.p);
+		                     b->ninst, b->d_grad.p, b->d_rgrad.p, b->d_ughost.p);
 	});
 }
 
@@ -526,7 +527,7 @@ fvx_status fvx_batch_residual_norms(fvx_batch b, const double* u, double* out, v
 		ba.list = b->d_all.p;
 		ba.gp = b->d_gas.p;
 		fvx::launch_residual(d, u, b->n_r.p, 0, nullptr, s, ba, b->ninst, b->d_grad.p,
-		                     b->d_rgrad.p);
+		                     b->d_rgrad.p, b->d_ughost.p);
 		fvx::launch_resnorm(n, b->n_r.p, d.m->d_area.p, b->n_part.p, b->n_out.p, 1, L, s);
 		FVX_HIP(hipMemcpyAsync(b->n_host, b->n_out.p, b->ninst*sizeof(double),
 		                       hipMemcpyDeviceToHost, s));
@@ -550,7 +551,7 @@ fvx_status fvx_batch_functionals(fvx_batch b, const double* u, int wall_tag, dou
 			fvx::BatchArg ba;
 			ba.list = b->d_all.p + i;
 			ba.gp = b->d_gas.p;
-			fvx::launch_gradients(d, u, b->d_grad.p, nullptr, 1, s, ba, 1);
+			fvx::launch_gradients(d, u, b->d_grad.p, nullptr, 1, s, ba, 1, b->d_ughost.p);
 			fvx::launch_functionals(d, u + 4*n*i, b->d_grad.p + 8*n*i, wall_tag, b->d_scratch.p, s,
 			                        &b->gas[i], b->aoa[i]);
 			FVX_HIP(hipMemcpyAsync(part.data(), b->d_scratch.p, sizeof(double)*part.size(),

@@ -153,6 +153,7 @@ struct fvx_disc_s {
 	fvx::DBuf<double> d_gcoef;             // [nelem][4][2] gradient stencil coefficients
 	fvx::DBuf<double> d_eps2;              // [nelem] Venkatakrishnan (K h)^3
 	fvx::DBuf<double> d_grad;              // [nelem][8] unlimited primitive gradients
+	fvx::DBuf<double> d_ughost;            // [nbface][4] ghost states of the boundary faces
 	fvx::DBuf<double> d_rgrad;             // [nelem][8] reconstruction gradients (limited)
 	fvx::DBuf<double> d_scratch;           // reductions for functionals
 	fvx::DBuf<double> mf_aux, mf_raux;     // [nelem][4] matrix-free state and residual, first use
@@ -171,6 +172,7 @@ struct fvx_batch_s {
 	fvx::DBuf<fvx::GasParams> d_gas;
 	fvx::DBuf<int> d_all;                  // 0 .. ninst-1
 	fvx::DBuf<double> d_grad, d_rgrad;     // [ninst][nelem][8]
+	fvx::DBuf<double> d_ughost;            // [ninst][nbface][4]
 	fvx::DBuf<double> d_scratch;           // functional partial sums
 	fvx::DBuf<double> n_r, n_part, n_out;  // residual-norm scratch, allocated on first use
 	double* n_host = nullptr;              // pinned host copy of the norms
@@ -249,14 +251,16 @@ GasParams make_gas(const fvx_physics_cfg& p);
 
 // kernel launchers (spatial.hip, flux_inst.hip); a batched launch passes the instance list
 // and per-instance parameters in BatchArg and the number of instances in nact
+/// ughost: the boundary faces' ghost states, [instance][nbface][4] for a batched launch (null:
+/// the discretisation's buffer, for a launch of instance 0 only, like grad and rgrad).
 void launch_gradients(const fvx_disc_s& d, const double* u, double* grad, double* rgrad,
                       int conserved, hipStream_t s, const BatchArg& ba = BatchArg(),
-                      int nact = 1);
+                      int nact = 1, double* ughost = nullptr);
 /// Whether the wave-task form of the flux pass covers this discretisation's options.
 bool flux_wave_apply(const fvx_disc_s& d);
 void launch_residual(const fvx_disc_s& d, const double* u, double* r, int gettimesteps,
                      double* dt, hipStream_t s, const BatchArg& ba = BatchArg(), int nact = 1,
-                     double* grad = nullptr, double* rgrad = nullptr);
+                     double* grad = nullptr, double* rgrad = nullptr, double* ughost = nullptr);
 void launch_functionals(const fvx_disc_s& d, const double* u, const double* cgrad, int walltag,
                         double* dev_out, hipStream_t s, const GasParams* gp = nullptr,
                         double aoa = 0.0);

@@ -719,6 +719,7 @@ void build_disc(fvx_disc_s& d)
 	d.d_eps2.upload(eps2);
 	d.d_grad.alloc((size_t)ne*8);
 	d.d_rgrad.alloc((size_t)ne*8);
+	d.d_ughost.alloc((size_t)std::max(m.nbface, 1)*4);
 	d.d_scratch.alloc(4096);
 	// the wave-task flux form where it applies and the cell order packs at least 24 cells per
 	// wave64 group on average (compact orders: Hilbert, structured natural orders); it spends

@@ -578,14 +578,15 @@ static void matfree_batch(const fvx_disc_s& d, const double* u, const double* ru
                           const double* dtm, const double* x, long sx, double* y, long sy,
                           double eps, double* aux, double* raux,
                           double* partial, double* xx, long sxx, const Inst& L,
-                          const BatchArg& ba, double* grad, double* rgrad, hipStream_t s)
+                          const BatchArg& ba, double* grad, double* rgrad, double* ughost,
+                          hipStream_t s)
 {
 	const int n = d.m->nelem;
 	const long N = 4L*n;
 	launch_mdot(N, x, 0, sx, 1, x, sx, partial, xx, sxx, L, s);
 	launch_mf_prep(N, u, x, sx, xx, sxx, eps, aux, L, s);
 	launch_zero(N, raux, N, L, s);
-	launch_residual(d, aux, raux, 0, nullptr, s, ba, L.nact, grad, rgrad);
+	launch_residual(d, aux, raux, 0, nullptr, s, ba, L.nact, grad, rgrad, ughost);
 	launch_mf_finish(n, x, sx, dtm, ru, raux, xx, sxx, eps, y, sy, L, s);
 }
 
@@ -595,7 +596,7 @@ void matfree_apply(const fvx_disc_s& d, const double* u, const double* ru, const
 {
 	const long N = 4L*d.m->nelem;
 	matfree_batch(d, u, ru, dtm, x, N, y, N, eps, aux, raux, partial, xx, 0, Inst(),
-	              BatchArg(), nullptr, nullptr, s);
+	              BatchArg(), nullptr, nullptr, nullptr, s);
 }
 
 /* ------------------------------------------------------------ steady solver ------------ */
@@ -604,7 +605,7 @@ struct SolverWork {
 	int n = -1, ninst = 0, jprec = -1, pctype = -1, pcprec = -1;
 	fvx_matrix_s A;
 	fvx_pc_s P;
-	DBuf<double> r, dt, dtm, du, aux, raux, grad, rgrad, cfl;
+	DBuf<double> r, dt, dtm, du, aux, raux, grad, rgrad, ughost, cfl;
 	double* cfl_pin = nullptr;
 	Krylov kry;
 	std::vector<fvx_step_record> hist;     // per-step record of the last one-instance solve
@@ -658,7 +659,10 @@ void steady_core(fvx_disc_s& d, const fvx_solver_cfg& c, double* u, int ninst,
 	if(w.n != n || w.ninst != ninst) {
 		w.r.alloc(N*ninst); w.dt.alloc((size_t)n*ninst); w.dtm.alloc((size_t)n*ninst);
 		w.du.alloc(N*ninst); w.aux.alloc(N*ninst); w.raux.alloc(N*ninst);
-		if(!single) { w.grad.alloc(8L*n*ninst); w.rgrad.alloc(8L*n*ninst); }
+		if(!single) {
+			w.grad.alloc(8L*n*ninst); w.rgrad.alloc(8L*n*ninst);
+			w.ughost.alloc(4L*std::max(m.nbface, 1)*ninst);
+		}
 		w.cfl.alloc(ninst);
 		if(w.cfl_pin) (void)hipHostFree(w.cfl_pin);
 		w.cfl_pin = nullptr;
@@ -691,6 +695,7 @@ void steady_core(fvx_disc_s& d, const fvx_solver_cfg& c, double* u, int ninst,
 	K.ensure(N, c.implicit ? c.krylov_restart : 1, ninst);
 	double* grad = single ? nullptr : w.grad.p;
 	double* rgrad = single ? nullptr : w.rgrad.p;
+	double* ughost = single ? nullptr : w.ughost.p;
 
 	struct Ist { double cfl = 0, resi = 1, resiold = 1, initres = 1; int step = 0; };
 	std::vector<Ist> is(ninst);
@@ -716,7 +721,7 @@ void steady_core(fvx_disc_s& d, const fvx_solver_cfg& c, double* u, int ninst,
 		BatchArg ba;
 		if(!single) { ba.list = L.list; ba.gp = gp; ba.cfl = w.cfl.p; }
 		FVX_HIP(hipMemsetAsync(w.r.p, 0, (size_t)N*ninst*sizeof(double), s));
-		launch_residual(d, u, w.r.p, 1, w.dt.p, s, ba, L.nact, grad, rgrad);
+		launch_residual(d, u, w.r.p, 1, w.dt.p, s, ba, L.nact, grad, rgrad, ughost);
 		launch_resnorm(n, w.r.p, m.d_area.p, K.partial.p, K.scal.p + Krylov::RES, Krylov::NSCAL,
 		               L, s);
 		if(c.implicit) {
@@ -758,7 +763,7 @@ void steady_core(fvx_disc_s& d, const fvx_solver_cfg& c, double* u, int ninst,
 					bb.list = LL.list;
 					matfree_batch(d, u, w.r.p, w.dtm.p, x, sx, y, sy, c.mf_eps, w.aux.p, w.raux.p,
 					              K.partial.p, K.scal.p + Krylov::XX,
-					              Krylov::NSCAL, LL, bb, grad, rgrad, s);
+					              Krylov::NSCAL, LL, bb, grad, rgrad, ughost, s);
 				};
 			else
 				op = [&](const double* x, long sx, double* y, long sy, const Inst& LL) {

@@ -32,13 +32,41 @@ FVX_DECL_FLUXW(0) FVX_DECL_FLUXW(1) FVX_DECL_FLUXW(2) FVX_DECL_FLUXW(3) FVX_DECL
 FVX_DECL_FLUXW(5) FVX_DECL_FLUXW(6)
 #undef FVX_DECL_FLUXW
 
+/// Ghost states of the boundary faces from their cells' states (the reference's
+/// compute_boundary_states, src/spatial/flow_spatial.cpp:74-82, called at :679 before the
+/// gradients): one thread per boundary face ib, the state at ughost[ib], instance-major like
+/// the state.
+__global__ __launch_bounds__(256)
+void k_ghost(int ncell, int nbface, const double* __restrict__ u, const int* __restrict__ bhost,
+             const FaceGeom* __restrict__ fgeom, const int* __restrict__ bcid,
+             const GasParams* __restrict__ gpa, const BCTable* __restrict__ bta,
+             const BatchArg ba, double* __restrict__ ughost)
+{
+	const long I = ba.list ? ba.list[blockIdx.y] : 0;
+	if(I < 0) return;                     // a retired slot of a compacted list
+	const int ib = blockIdx.x*blockDim.x + threadIdx.x;
+	if(ib >= nbface) return;
+	u += I*4L*ncell;
+	ughost += I*4L*nbface;
+	double uc[4], g[4];
+	load4(u + 4*bhost[ib], uc);
+	// boundary face ib is face ib (boundary faces are numbered first)
+	const FaceGeom G = fgeom[ib];
+	const double n[2] = { G.nx, G.ny };
+	ghost_state(gpa[I], *bta, bcid[ib], uc, n, g);
+	store4(ughost + 4*ib, g);
+}
+
 /// LIM: 0 none, 1 Venkatakrishnan, 2 Barth-Jespersen. CONS: gradients of the conserved
 /// variables (getGradients) instead of the primitive ones. gout receives the unlimited
 /// gradient (may be null when LIM > 0 and it is not needed), rout the limited one.
 /// Two lanes per cell, each owning two of the cell's local faces; the pair combines its
 /// partial sums, bounds and limiter values with one DPP exchange. A workgroup owns a range of
 /// consecutive cells and first stages their primitive states in LDS, so a neighbour inside the
-/// range is read from there instead of being fetched and converted again.
+/// range is read from there instead of being fetched and converted again. A boundary face's
+/// neighbour value is its ghost state, computed beforehand by k_ghost: built inline, the ghost
+/// state's registers were live beside both neighbour gathers (118 VGPRs, 4 waves per SIMD,
+/// against 79 and 6 without it).
 ///
 /// The limiter factor of a cell and variable is the minimum over its faces of a ratio N/D with
 /// D >= 0 (Venkatakrishnan: N = dp^2 + 2 dp dm + eps^2, D = dp^2 + dp dm + 2 dm^2 + eps^2;
@@ -50,20 +78,18 @@ FVX_DECL_FLUXW(5) FVX_DECL_FLUXW(6)
 template <int LIM, bool CONS>
 __global__ __launch_bounds__(MAX_BLOCK)
 void k_gradients(int ncell, const double* __restrict__ u, const int* __restrict__ nbr,
-                 const FaceGeom* __restrict__ fgeom,
                  const double* __restrict__ soff, const double* __restrict__ gcoef,
-                 const double* __restrict__ eps2, const int* __restrict__ bcid,
-                 const GasParams* __restrict__ gpa, const BCTable* __restrict__ bta,
+                 const double* __restrict__ eps2, const GasParams* __restrict__ gpa,
                  const BatchArg ba, double* __restrict__ gout, double* __restrict__ rout,
-                 const int xcd)
+                 const int xcd, int nbface, const double* __restrict__ ughost)
 {
 	// instance: state and gradients are instance-major, 4 and 8 values per cell; gpa holds
 	// the gas parameters of each instance (one entry for a single solve)
 	const long I = ba.list ? ba.list[blockIdx.y] : 0;
 	if(I < 0) return;                     // a retired slot of a compacted list
 	const GasParams& gp = gpa[I];
-	const BCTable& bt = *bta;
 	u += I*4L*ncell;
+	ughost += I*4L*nbface;
 	if(gout) gout += I*8L*ncell;
 	if(rout) rout += I*8L*ncell;
 	const int p = threadIdx.x & 1;
@@ -90,8 +116,12 @@ void k_gradients(int ncell, const double* __restrict__ u, const int* __restrict_
 	double uj[2][4];
 #pragma unroll
 	for(int k = 0; k < 2; k++) {
-		outside[k] = nbs[k] >= 0 && (unsigned)(nbs[k] - c0) >= (unsigned)nloc;
-		load4(u + 4*(outside[k] ? nbs[k] : cc), uj[k]);
+		const int nb = nbs[k];
+		outside[k] = nb >= 0 && (unsigned)(nb - c0) >= (unsigned)nloc;
+		// boundary face ib's ghost state (k_ghost) is at ughost[ib]
+		const double* src = nb >= 0 ? u + 4*(outside[k] ? nb : cc)
+		                  : nb != SLOT_UNUSED ? ughost + 4*(-1 - nb) : u + 4*cc;
+		load4(src, uj[k]);
 	}
 	const double2 c0v = cf[0], c1v = cf[1];
 	// face midpoint minus cell centre of the two slots (zero for an unused slot)
@@ -113,12 +143,6 @@ void k_gradients(int ncell, const double* __restrict__ u, const int* __restrict_
 		if(nb >= 0 && !outside[k]) {
 			for(int v = 0; v < 4; v++) wj[v] = sw[v][nb - c0];
 		} else {
-			if(nb < 0) {
-				// boundary face ib is face ib (boundary faces are numbered first)
-				const FaceGeom G = fgeom[-1 - nb];
-				const double n[2] = { G.nx, G.ny };
-				ghost_state(gp, bt, bcid[-1 - nb], ui, n, uj[k]);
-			}
 			if constexpr(CONS) { for(int v = 0; v < 4; v++) wj[v] = uj[k][v]; }
 			else prim_lean(gp, uj[k], wj);
 		}
@@ -273,18 +297,29 @@ Profiler::~Profiler()
 }
 
 void launch_gradients(const fvx_disc_s& d, const double* u, double* grad, double* rgrad,
-                      int conserved, hipStream_t s, const BatchArg& ba, int nact)
+                      int conserved, hipStream_t s, const BatchArg& ba, int nact, double* ughost)
 {
 	const fvx_mesh_s& m = *d.m;
+	if(!ughost) {
+		// the discretisation's buffer holds one instance (a single-instance launch, whose list,
+		// if any, names instance 0)
+		if(nact != 1)
+			throw Error(FVX_ERR_INVALID, "a batched gradient launch needs its ghost-state buffer");
+		ughost = d.d_ughost.p;
+	}
 	const int block = d.tune.grad_block, cpb = block/2;
 	const int nblk = (m.nelem + cpb - 1)/cpb;
 	const int lim = conserved ? 0 : (d.nc.reconstruction == FVX_RECON_VENKATAKRISHNAN ? 1 :
 	                                 d.nc.reconstruction == FVX_RECON_BARTHJESPERSEN ? 2 : 0);
 #define FVX_GL(L, C) hipLaunchKernelGGL((k_gradients<L, C>), dim3(nblk, nact), dim3(block), 0, \
-		s, m.nelem, u, m.d_nbr.p, m.d_fgeom.p, m.d_soff.p, d.d_gcoef.p, \
-		d.d_eps2.p, d.d_bcid.p, ba.gp ? ba.gp : d.d_gas.p, d.d_bct.p, ba, grad, rgrad, \
-		d.tune.grad_xcd)
+		s, m.nelem, u, m.d_nbr.p, m.d_soff.p, d.d_gcoef.p, \
+		d.d_eps2.p, ba.gp ? ba.gp : d.d_gas.p, ba, grad, rgrad, \
+		d.tune.grad_xcd, m.nbface, ughost)
 	d.prof.start(PROF_GRADIENTS, s);
+	if(m.nbface > 0)
+		hipLaunchKernelGGL(k_ghost, dim3((m.nbface + 255)/256, nact), dim3(256), 0, s, m.nelem,
+		                   m.nbface, u, m.d_bhost.p, m.d_fgeom.p, d.d_bcid.p,
+		                   ba.gp ? ba.gp : d.d_gas.p, d.d_bct.p, ba, ughost);
 	if(conserved) FVX_GL(0, true);
 	else if(lim == 0) FVX_GL(0, false);
 	else if(lim == 1) FVX_GL(1, false);
@@ -303,7 +338,7 @@ bool flux_wave_apply(const fvx_disc_s& d)
 
 void launch_residual(const fvx_disc_s& d, const double* u, double* r, int gettimesteps,
                      double* dt, hipStream_t s, const BatchArg& ba, int nact, double* grad,
-                     double* rgrad)
+                     double* rgrad, double* ughost)
 {
 	const fvx_mesh_s& m = *d.m;
 	if(!grad) grad = d.d_grad.p;
@@ -317,14 +352,14 @@ void launch_residual(const fvx_disc_s& d, const double* u, double* r, int gettim
 	if(order2) {
 		const bool limited = rec == FVX_RECON_VENKATAKRISHNAN || rec == FVX_RECON_BARTHJESPERSEN;
 		if(limited) {
-			launch_gradients(d, u, visc ? grad : nullptr, rgrad, 0, s, ba, nact);
+			launch_gradients(d, u, visc ? grad : nullptr, rgrad, 0, s, ba, nact, ughost);
 			rg = rgrad; ug = grad; mode = 1;
 		} else if(rec == FVX_RECON_WENO) {
-			launch_gradients(d, u, grad, nullptr, 0, s, ba, nact);
+			launch_gradients(d, u, grad, nullptr, 0, s, ba, nact, ughost);
 			launch_weno(d, grad, rgrad, s, ba, nact);
 			rg = rgrad; ug = grad; mode = 1;
 		} else {
-			launch_gradients(d, u, grad, nullptr, 0, s, ba, nact);
+			launch_gradients(d, u, grad, nullptr, 0, s, ba, nact, ughost);
 			rg = grad; ug = grad;
 			mode = rec == FVX_RECON_VANALBADA ? 2 : 1;
 		}
