@@ -643,7 +643,11 @@ __device__ __forceinline__ double block_sum(double v, double* sh)
 	return (sh[0] + sh[1]) + (sh[2] + sh[3]);
 }
 
-/// partial[b][i] = block b's part of dot(V_i, w), i < nv (V_i = V + i*ldv).
+/// partial[b][i] = block b's part of dot(V_i, w), i < nv <= NVM (V_i = V + i*ldv). The NVM
+/// accumulators are indexed by compile-time constants only, so they stay in registers; each
+/// wave reduces all of them, then one barrier, then thread i adds the four waves' parts of dot
+/// i in the fixed order (w0 + w1) + (w2 + w3).
+template <int NVM>
 __global__ __launch_bounds__(BLOCK)
 void k_mdot(long N, const int* __restrict__ ilist, const double* __restrict__ V, long ldv,
             long sv, int nv, const double* __restrict__ w, long sw, double* __restrict__ partial)
@@ -651,20 +655,31 @@ void k_mdot(long N, const int* __restrict__ ilist, const double* __restrict__ V,
 	const long I = inst_of(ilist);
 	if(I < 0) return;                     // a retired slot of a compacted list
 	V += I*sv; w += I*sw; partial += I*PART;
-	__shared__ double sh[4];
-	double acc[MAX_DOTS];
-	for(int i = 0; i < MAX_DOTS; i++) acc[i] = 0.0;
+	static_assert(BLOCK == 256, "the final sum adds four waves' parts");
+	__shared__ double sh[BLOCK/64][NVM];
+	double acc[NVM];
+#pragma unroll
+	for(int i = 0; i < NVM; i++) acc[i] = 0.0;
 	const long stride = (long)gridDim.x*blockDim.x;
 	for(long e = (long)blockIdx.x*blockDim.x + threadIdx.x; e < N; e += stride) {
 		const double we = w[e];
-		for(int i = 0; i < MAX_DOTS; i++)
+#pragma unroll
+		for(int i = 0; i < NVM; i++)
 			if(i < nv) acc[i] += V[i*ldv + e]*we;
 	}
-	for(int i = 0; i < MAX_DOTS; i++) {
-		if(i >= nv) break;
-		const double s = block_sum(acc[i], sh);
-		if(threadIdx.x == 0) partial[(size_t)blockIdx.x*MAX_DOTS + i] = s;
+	const int wv = threadIdx.x >> 6;
+#pragma unroll
+	for(int i = 0; i < NVM; i++) {
+		if(i < nv) {
+			double v = acc[i];
+			for(int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+			if((threadIdx.x & 63) == 0) sh[wv][i] = v;
+		}
 	}
+	__syncthreads();
+	const int t = threadIdx.x;
+	if(t < nv)
+		partial[(size_t)blockIdx.x*MAX_DOTS + t] = (sh[0][t] + sh[1][t]) + (sh[2][t] + sh[3][t]);
 }
 
 /// out[i] = sum over blocks of partial[b][i], i < nv (one block per instance, fixed order).
@@ -1115,8 +1130,13 @@ long partial_stride() { return PART; }
 void launch_mdot(long N, const double* V, long ldv, long sv, int nv, const double* w, long sw,
                  double* partial, double* out, long so, const Inst& L, hipStream_t s)
 {
-	hipLaunchKernelGGL(k_mdot, dim3(RED_BLOCKS, L.nact), dim3(BLOCK), 0, s, N, L.list, V, ldv, sv,
-	                   nv, w, sw, partial);
+	if(nv < 1 || nv > MAX_DOTS) throw Error(FVX_ERR_INVALID, "multi-dot of 1 to 32 vectors");
+#define FVX_MD(M) hipLaunchKernelGGL(k_mdot<M>, dim3(RED_BLOCKS, L.nact), dim3(BLOCK), 0, s, N, \
+	L.list, V, ldv, sv, nv, w, sw, partial)
+	if(nv <= 8) FVX_MD(8);
+	else if(nv <= 16) FVX_MD(16);
+	else FVX_MD(MAX_DOTS);
+#undef FVX_MD
 	hipLaunchKernelGGL(k_reduce, dim3(1, L.nact), dim3(BLOCK), 0, s, RED_BLOCKS, L.list, partial,
 	                   nv, out, so);
 	FVX_HIP(hipGetLastError());
