@@ -187,7 +187,8 @@ PC_NAMES = {"sgs": "multicolour SGS", "ilu0": "multicolour block ILU(0)",
             "bjacobi_ilu0": "block Jacobi + ILU(0)"}
 
 
-def gpu_time_to_tol(name, tol, precision, sweeps=16, pc="sgs", subdomain=256, order="rcm"):
+def gpu_time_to_tol(name, tol, precision, sweeps=16, pc="sgs", subdomain=256, order="rcm",
+                    pc_precision=None):
     """Pseudo-time solve from free stream (first-order starter, then the main solve) until
     sqrt(sum r_E^2 area) <= tol x the same norm at the free-stream state."""
     import numpy as np
@@ -204,7 +205,7 @@ def gpu_time_to_tol(name, tol, precision, sweeps=16, pc="sgs", subdomain=256, or
     d2.compute_residual(uinf, r, True, dt)
     _, _, area, _ = mesh.faces()
     norm0 = float(np.sqrt((r[:, 3].cpu().numpy() ** 2 * area).sum()))
-    kw = dict(min_relax=0.2, pc=pc, pc_sweeps=sweeps, pc_precision=precision,
+    kw = dict(min_relax=0.2, pc=pc, pc_sweeps=sweeps, pc_precision=pc_precision or precision,
               jac_precision=precision, pc_subdomain_cells=subdomain, raise_on_failure=False)
     u = uinf.clone()
     torch.cuda.synchronize()
@@ -222,6 +223,7 @@ def gpu_time_to_tol(name, tol, precision, sweeps=16, pc="sgs", subdomain=256, or
                 starter_steps=s1["steps"], steps=s2["steps"], lin_iters=s2["lin_iters"],
                 final_rel=s2["final_rel"], converged=bool(s2["converged"]),
                 CL=f["CL"], CDp=f["CDp"], precision=precision,
+                pc_precision=pc_precision or precision,
                 pc=f"{PC_NAMES[pc]} x{sweeps}"
                    + (f" ({subdomain}-cell subdomains)" if pc == "bjacobi_ilu0" else ""))
 
@@ -300,7 +302,7 @@ def cpu_time_to_tol(name, tol):
                 CL=f["CL"], CDp=f["CDp"], threads=1, pc="natural-order block ILU(0)")
 
 
-def batch_sweep(ninst, maxiter, sweeps=16, tol=1e-8):
+def batch_sweep(ninst, maxiter, sweeps=16, tol=1e-8, pc_precision="f32"):
     """Batched operating-point sweep on the 98,304-cell airfoil: ninst seeded points
     (std::mt19937_64 seed 4242, Mach U[0.3, 0.75], AoA U[-4, 6] deg), Roe + WLS +
     Venkatakrishnan K=5, first-order starter then the second-order matrix-free solve, FP32
@@ -315,6 +317,7 @@ def batch_sweep(ninst, maxiter, sweeps=16, tol=1e-8):
     gm = fvx.Mesh(raw)
     d1, d2 = tw.discretisations(gm)
     pc32, _, starter, main = tw.solver_settings(sweeps, maxiter, tol)
+    pc32 = dict(pc32, pc_precision=pc_precision)
     mach, aoa = operating_points(4242, ninst)
     b = fvx.Batch(d2, mach, aoa)
     norms = b.residual_norms(ws.freestream_states(b))
@@ -329,7 +332,8 @@ def batch_sweep(ninst, maxiter, sweeps=16, tol=1e-8):
     torch.cuda.empty_cache()
     return {
         "case": f"{ninst} operating points on the 98,304-cell airfoil, Roe, WLS, "
-                f"Venkatakrishnan K=5, FP32 Jacobian + multicolour SGS x{sweeps}, FP64 residual",
+                f"Venkatakrishnan K=5, FP32 Jacobian + multicolour SGS x{sweeps} "
+                f"({pc_precision.upper()} factors), FP64 residual",
         "value": round(cell_iters / wall / 1e6, 2), "unit": "Mcell-iterations/s (aggregate)",
         "seconds": round(wall, 3), "starter_seconds": round(o["t_starter"], 3),
         "tol": tol, "maxiter": maxiter,
@@ -360,8 +364,13 @@ def warm_start_ab(params_path, ntest, nsingle, maxiter, tol=1e-8, sweeps=16, cas
                     pre_steps=pre)
     single = tw.single_ab(net, d1, d2, mach[:nsingle], aoa[:nsingle], pc32, pc64, starter,
                           main, quiet, wcfl, pre_steps=pre)
+    # the same warm arm with bfloat16 preconditioner factors over the FP32 Jacobian
+    w16 = tw.warm_arm(net, d1, d2, mach, aoa, dict(pc32, pc_precision="bf16"), pc64, starter,
+                      main, wcfl, pre_steps=pre)
     pts = ab["points"]
     ok = [p for p in pts if p["cold_status"] == 0 and p["warm_status"] == 0]
+    ok16 = [(p, q) for p, q in zip(pts, w16["points"]) if p["cold_status"] == 0
+            and q["status"] == 0]
     return {
         "case": (f"{ntest} held-out points (seed 200) on the 98,304-cell airfoil, Roe, WLS, "
                  f"Venkatakrishnan K=5" if case == "airfoil" else
@@ -377,7 +386,12 @@ def warm_start_ab(params_path, ntest, nsingle, maxiter, tol=1e-8, sweeps=16, cas
                     "warm_steps": [p["warm_steps"] for p in pts],
                     "both_converged": len(ok),
                     "max_dCL": max([p["dCL"] for p in ok], default=None),
-                    "max_dCDp": max([p["dCDp"] for p in ok], default=None)},
+                    "max_dCDp": max([p["dCDp"] for p in ok], default=None),
+                    "warm_bf16_pc_seconds": round(w16["t"], 3),
+                    "speedup_bf16_pc": round(ab["t_cold"] / w16["t"], 3),
+                    "fallbacks_bf16_pc": w16["fallbacks"],
+                    "max_dCL_bf16_pc": max([abs(q["CL"] - p["CL"]) for p, q in ok16],
+                                           default=None)},
         "single": [{k: (float("%.4g" % v) if isinstance(v, float) else v) for k, v in r.items()}
                    for r in single],
         "single_speedup": round(sum(r["t_cold"] for r in single)
@@ -498,6 +512,9 @@ def time_to_tol(a):
     bump32 = gpu_time_to_tol("bump", 1e-10, "f32", sweeps=12)
     air64 = gpu_time_to_tol("airfoil", 1e-8, "f64", sweeps=24)
     air32 = gpu_time_to_tol("airfoil", 1e-8, "f32", sweeps=24)
+    # bfloat16 preconditioner factors over the FP32 Jacobian
+    bump16 = gpu_time_to_tol("bump", 1e-10, "f32", sweeps=12, pc_precision="bf16")
+    air16 = gpu_time_to_tol("airfoil", 1e-8, "f32", sweeps=24, pc_precision="bf16")
     tt = {
         "criterion": "sqrt(sum r_energy^2 area) <= tol x the same norm at the free stream",
         "bump_1e-10": {
@@ -505,6 +522,8 @@ def time_to_tol(a):
                     "unlimited; first-order starter then main solve",
             "gpu_fp64": bump64, "gpu_fp32_jacobian_and_pc": bump32,
             "fp32_speedup": round(bump64["seconds"] / bump32["seconds"], 3),
+            "gpu_fp32_jacobian_bf16_pc": bump16,
+            "bf16_pc_speedup": round(bump64["seconds"] / bump16["seconds"], 3),
             "warm_start": None},
         "airfoil_1e-8": {
             "case": "NACA 4-digit O-mesh 512x128, 98,304 cells, M0.5, alpha 1.25, HLLC, WLS, "
@@ -513,7 +532,11 @@ def time_to_tol(a):
             "gpu_fp64": air64, "gpu_fp32_jacobian_and_pc": air32,
             "fp32_speedup": round(air64["seconds"] / air32["seconds"], 3),
             "CL_CDp_fp32_vs_fp64": [abs(air32["CL"] / air64["CL"] - 1),
-                                    abs(air32["CDp"] / air64["CDp"] - 1)]},
+                                    abs(air32["CDp"] / air64["CDp"] - 1)],
+            "gpu_fp32_jacobian_bf16_pc": air16,
+            "bf16_pc_speedup": round(air64["seconds"] / air16["seconds"], 3),
+            "CL_CDp_bf16_pc_vs_fp64": [abs(air16["CL"] / air64["CL"] - 1),
+                                       abs(air16["CDp"] / air64["CDp"] - 1)]},
     }
     if not a.no_cpu_baseline:
         tt["bump_1e-10"]["cpu_oracle_fp64"] = cpu_time_to_tol("bump", 1e-10)
@@ -643,6 +666,7 @@ def main():
         out["time_to_tol"] = time_to_tol(a)
     if not a.no_batch:
         out["batch"] = batch_sweep(a.batch_instances, a.batch_maxiter)
+        out["batch_bf16_pc"] = batch_sweep(a.batch_instances, a.batch_maxiter, pc_precision="bf16")
         out["mpnn"] = mpnn_timing(8, 20)
     print(json.dumps(out), flush=True)
 

@@ -23,6 +23,9 @@ struct Error : std::runtime_error {
 /// Marker in a cell's neighbour slot that is not used (triangles have 3 faces).
 constexpr int SLOT_UNUSED = -0x40000000;
 
+/// bfloat16 storage of a preconditioner factor (the upper 16 bits of an IEEE binary32).
+struct bf16 { unsigned short bits; };
+
 /// Face geometry record: unit normal pointing from the left (lower) cell to the right cell,
 /// length, midpoint, and the two cells: left, and right or -1-ib for boundary face ib (48 bytes,
 /// three 16-byte loads).
@@ -226,9 +229,11 @@ struct fvx_pc_s {
 	fvx::DBuf<int4> d_ploc;
 	fvx::DBuf<double> v64;
 	fvx::DBuf<float> v32;
+	fvx::DBuf<fvx::bf16> v16;
 	fvx::Krylov* kry = nullptr;            // FGMRES workspace of fvx_linear_solve, first use
 	double* f64() const { return v64.p; }
 	float* f32() const { return v32.p; }
+	fvx::bf16* fb16() const { return v16.p; }
 	~fvx_pc_s() { fvx::destroy_krylov(kry); }
 };
 

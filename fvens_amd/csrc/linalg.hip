@@ -34,7 +34,30 @@ template <> __device__ __forceinline__ void load_row<float>(const float* p, doub
 	const float4 a = *reinterpret_cast<const float4*>(p);
 	v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
 }
+/// bfloat16 rows: 4 x 2 bytes (8-byte aligned: rows start at multiples of 4 values)
+__device__ __forceinline__ double bf16_value(unsigned int h)
+{
+	return (double)__uint_as_float(h << 16);
+}
+template <> __device__ __forceinline__ void load_row<bf16>(const bf16* p, double* v)
+{
+	const uint2 w = *reinterpret_cast<const uint2*>(p);
+	v[0] = bf16_value(w.x & 0xffffu); v[1] = bf16_value(w.x >> 16);
+	v[2] = bf16_value(w.y & 0xffffu); v[3] = bf16_value(w.y >> 16);
+}
+/// FP64 -> FP32 (rounded) -> bfloat16, rounded to nearest even; NaN stays NaN
+__device__ __forceinline__ unsigned int bf16_bits(double x)
+{
+	const unsigned int u = __float_as_uint((float)x);
+	if((u & 0x7fffffffu) > 0x7f800000u) return (u >> 16) | 0x40u;
+	return (u + 0x7fffu + ((u >> 16) & 1u)) >> 16;
+}
 template <class VT> __device__ __forceinline__ void put_row(VT* p, const double* v);
+template <> __device__ __forceinline__ void put_row<bf16>(bf16* p, const double* v)
+{
+	*reinterpret_cast<uint2*>(p) = make_uint2(bf16_bits(v[0]) | (bf16_bits(v[1]) << 16),
+	                                          bf16_bits(v[2]) | (bf16_bits(v[3]) << 16));
+}
 template <> __device__ __forceinline__ void put_row<double>(double* p, const double* v)
 {
 	reinterpret_cast<double2*>(p)[0] = make_double2(v[0], v[1]);
@@ -945,9 +968,11 @@ void launch_pc_setup(fvx_pc_s& p, const fvx_matrix_s& A, const Inst& L, hipStrea
 	if(A.ninst < p.ninst) throw Error(FVX_ERR_INVALID, "matrix holds fewer instances than the preconditioner");
 	if(A.prec == FVX_PREC_F32) {
 		if(p.prec == FVX_PREC_F32) pc_setup_t(p, A.f32(), p.f32(), L, s);
+		else if(p.prec == FVX_PREC_BF16) pc_setup_t(p, A.f32(), p.fb16(), L, s);
 		else pc_setup_t(p, A.f32(), p.f64(), L, s);
 	} else {
 		if(p.prec == FVX_PREC_F32) pc_setup_t(p, A.f64(), p.f32(), L, s);
+		else if(p.prec == FVX_PREC_BF16) pc_setup_t(p, A.f64(), p.fb16(), L, s);
 		else pc_setup_t(p, A.f64(), p.f64(), L, s);
 	}
 }
@@ -1120,6 +1145,7 @@ void launch_pc_apply(const fvx_pc_s& p, const double* r, long sr, double* z, lon
 	}
 	if(!p.A) throw Error(FVX_ERR_INVALID, "preconditioner applied before its set-up");
 	if(p.prec == FVX_PREC_F32) pc_apply_t(p, p.f32(), r, sr, z, sz, L, s);
+	else if(p.prec == FVX_PREC_BF16) pc_apply_t(p, p.fb16(), r, sr, z, sz, L, s);
 	else pc_apply_t(p, p.f64(), r, sr, z, sz, L, s);
 }
 

@@ -304,7 +304,7 @@ void pc_init(fvx_pc_s& p, const fvx_mesh_s* m, int type, int prec, int ninst)
 {
 	if(type < FVX_PC_NONE || type > FVX_PC_BJACOBI_ILU0)
 		throw Error(FVX_ERR_INVALID, "unknown preconditioner type");
-	if(prec != FVX_PREC_F64 && prec != FVX_PREC_F32)
+	if(prec != FVX_PREC_F64 && prec != FVX_PREC_F32 && prec != FVX_PREC_BF16)
 		throw Error(FVX_ERR_INVALID, "unknown preconditioner precision");
 	p.m = m;
 	p.type = type;
@@ -392,6 +392,9 @@ void pc_init(fvx_pc_s& p, const fvx_mesh_s* m, int type, int prec, int ninst)
 		if(prec == FVX_PREC_F32) {
 			p.v32.alloc(cnt);
 			FVX_HIP(hipMemset(p.v32.p, 0, cnt*sizeof(float)));
+		} else if(prec == FVX_PREC_BF16) {
+			p.v16.alloc(cnt);
+			FVX_HIP(hipMemset(p.v16.p, 0, cnt*sizeof(bf16)));
 		} else {
 			p.v64.alloc(cnt);
 			FVX_HIP(hipMemset(p.v64.p, 0, cnt*sizeof(double)));

@@ -367,7 +367,7 @@ class StepRecord(C.Structure):
                 ("cfl", C.c_double)]
 
 
-PREC = {"f64": 0, "f32": 1}
+PREC = {"f64": 0, "f32": 1, "bf16": 2}
 
 
 class TuneRow(C.Structure):

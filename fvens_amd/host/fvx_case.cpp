@@ -13,7 +13,7 @@
  * tests/flow_solve.cpp:89-123 (CL to 100 x tol, CDp and CDsf to tol, relative).
  *
  *   fvx_case FILE.ctrl [--mesh_file F] [--regression_file F] [--regression_tol 1e-8]
- *                      [--pc sgs] [--sweeps 16] [--precision f64|f32] [--matrix_free]
+ *                      [--pc sgs] [--sweeps 16] [--precision f64|f32|bf16] [--matrix_free]
  *                      [--matrix_free_difference_step 1e-7]
  *                      [--rcm] [--ksp_max_it 30] [--ksp_rtol 0.1] [--limiter_param 0]
  *                      [--mesh_reorder rcm|line|line_natural|line_rcm|hilbert]
@@ -28,7 +28,7 @@
  * testcases/naca0012/transonic-sanity-test-muscl.ctrl:
  *   fvx_case --mesh FILE [--mach 0.8] [--aoa 1.25] [--flux HLLC] [--gradient LEASTSQUARES]
  *            [--limiter VANALBADA] [--limiter-param 20] [--pc sgs] [--sweeps 1]
- *            [--precision f64|f32] [--tol 1e-7] [--maxiter 170] [--rcm]
+ *            [--precision f64|f32|bf16] [--tol 1e-7] [--maxiter 170] [--rcm]
  *            [--regression FILE] [--regression-tol 1e-8]
  */
 #include "fvens_gpu.hpp"
@@ -99,7 +99,9 @@ int run_control_file(const std::string& ctrl, std::map<std::string, std::string>
 
 	const FlowPhysicsConfig pconf = extract_spatial_physics_config(o);
 	FlowFV prob(&m, pconf, extract_spatial_numerics_config(o));
-	const int prec = opt["precision"] == "f32" ? FVX_PREC_F32 : FVX_PREC_F64;
+	// f64 | f32 | bf16 (bfloat16 preconditioner factors over an FP32 Jacobian)
+	const int prec = opt["precision"] == "f64" ? FVX_PREC_F64 : FVX_PREC_F32;
+	const int pc_prec = opt["precision"] == "bf16" ? FVX_PREC_BF16 : prec;
 	const size_t n = 4*(size_t)m.gnelem();
 	std::vector<double> uh(n);
 	const auto uinf = prob.compute_freestream_state();
@@ -109,7 +111,8 @@ int run_control_file(const std::string& ctrl, std::map<std::string, std::string>
 	SteadySolverConfig sc;
 	sc.pc_type = opt["pc"];
 	sc.pc_sweeps = std::stoi(opt["sweeps"]);
-	sc.pc_precision = sc.jac_precision = prec;
+	sc.pc_precision = pc_prec;
+	sc.jac_precision = prec;
 	sc.nl_update_scheme = o.nl_update_scheme.empty() ? "ROBUST_FLOW" : o.nl_update_scheme;
 	sc.min_nl_update = o.min_nl_update;
 	sc.ksp_max_it = std::stoi(opt["ksp_max_it"]);
@@ -181,7 +184,9 @@ int run_flags(std::map<std::string, std::string>& opt, bool rcm)
 	                              std::stod(opt["limiter-param"]), true};
 	FlowFV startprob(&m, pconf, start), prob(&m, pconf, main);
 
-	const int prec = opt["precision"] == "f32" ? FVX_PREC_F32 : FVX_PREC_F64;
+	// f64 | f32 | bf16 (bfloat16 preconditioner factors over an FP32 Jacobian)
+	const int prec = opt["precision"] == "f64" ? FVX_PREC_F64 : FVX_PREC_F32;
+	const int pc_prec = opt["precision"] == "bf16" ? FVX_PREC_BF16 : prec;
 	const size_t n = 4*(size_t)m.gnelem();
 	std::vector<double> uh(n);
 	const auto uinf = prob.compute_freestream_state();
@@ -191,7 +196,8 @@ int run_flags(std::map<std::string, std::string>& opt, bool rcm)
 	SteadySolverConfig sc;
 	sc.pc_type = opt["pc"];
 	sc.pc_sweeps = std::stoi(opt["sweeps"]);
-	sc.pc_precision = sc.jac_precision = prec;
+	sc.pc_precision = pc_prec;
+	sc.jac_precision = prec;
 	sc.cflinit = 50; sc.cflfin = 1000; sc.tol = 1e-1; sc.maxiter = 20;
 	SteadyBackwardEulerSolver starter(&startprob, sc);
 	try { starter.solve(u.p); }
@@ -235,6 +241,10 @@ int main(int argc, char** argv)
 			return 2;
 		}
 		opt[a.substr(2)] = argv[++i];
+	}
+	if(opt["precision"] != "f64" && opt["precision"] != "f32" && opt["precision"] != "bf16") {
+		std::fprintf(stderr, "--precision takes f64, f32 or bf16\n");
+		return 2;
 	}
 	try {
 		return ctrl.empty() ? run_flags(opt, rcm) : run_control_file(ctrl, opt, rcm, mf);

@@ -246,6 +246,35 @@ def ab_test(net, d1, d2, mach, aoa, pc32, pc64, starter, main, log, warm_cfl=100
     return out
 
 
+def warm_arm(net, d1, d2, mach, aoa, pcw, pc64, starter, main, warm_cfl=1000.0, chunk=64,
+             pre_steps=WARM_PRE_STEPS):
+    """The warm arm of ab_test alone with preconditioner settings pcw: warm-started solves in
+    batches of `chunk`, then the plain FP64 solve of the points whose warm start did not
+    converge. Returns the wall time, the fallback count and each point's functionals and
+    status (the fallback's where it ran)."""
+    t = 0.0
+    fallbacks = 0
+    pts = []
+    for c0 in range(0, len(mach), chunk):
+        m, a = mach[c0:c0 + chunk], aoa[c0:c0 + chunk]
+        b = fvx.Batch(d2, m, a)
+        norms = b.residual_norms(ws.freestream_states(b))
+        _, warm = warm_solve(net, d1, d2, m, a, pcw, main, norms, warm_cfl, pre_steps)
+        t += warm["t_inference"] + warm["t_pre"] + warm["t_main"]
+        res = [dict(status=w["status"], **f) for w, f in zip(warm["main"], warm["functionals"])]
+        failed = [i for i, s in enumerate(warm["main"]) if s["status"] != 0]
+        if failed:
+            _, fb = ws.solve_batch(d2, m[failed], a[failed], starter=d1,
+                                   starter_cfg=dict(starter, **pc64),
+                                   main_cfg=dict(main, **pc64), norms=norms[failed])
+            t += fb["t_starter"] + fb["t_main"]
+            for k, i in enumerate(failed):
+                res[i] = dict(status=fb["main"][k]["status"], **fb["functionals"][k])
+        fallbacks += len(failed)
+        pts += res
+    return dict(t=t, fallbacks=fallbacks, points=pts)
+
+
 def single_ab(net, d1, d2, mach, aoa, pc32, pc64, starter, main, log, warm_cfl=1000.0,
               pre_steps=WARM_PRE_STEPS):
     import torch

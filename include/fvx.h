@@ -160,8 +160,11 @@ fvx_status fvx_surface_data(fvx_disc d, const double* u, int wall_tag, double* o
 
 /* ---- implicit solve: system matrix, preconditioners, Krylov solver, pseudo-time loop ---- */
 
-/* Storage precision of matrices and preconditioner factors; vectors are always double. */
-enum { FVX_PREC_F64 = 0, FVX_PREC_F32 = 1 };
+/* Storage precision of matrices and preconditioner factors; vectors are always double.
+ * FVX_PREC_BF16 (preconditioner factors only: 8-bit mantissa, FP32's exponent range) halves
+ * the bytes a preconditioner application reads against FP32; the factors are computed in FP64
+ * and rounded to nearest even when stored. */
+enum { FVX_PREC_F64 = 0, FVX_PREC_F32 = 1, FVX_PREC_BF16 = 2 };
 /* Preconditioners (PETSc -pc_type none / pbjacobi / sor / bjacobi+ilu on one rank). SGS and
  * ILU0 run in a multicolour ordering of the cells (greedy colouring in cell order; cells of
  * one colour share no face), which makes each sweep of a colour fully parallel. */
