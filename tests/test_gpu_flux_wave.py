@@ -147,3 +147,41 @@ def test_the_form_is_selected_by_the_cell_order(torch_cuda):
                                                  bcs=[(2, "adiabaticwall"), (4, "farfield")]),
                    fvx.numerics_cfg(**num))
     assert d.get_launch("flux_wave")[0] == 0
+
+
+def test_full_size_hilbert_workload_matches_the_oracle(torch_cuda):
+    """The benchmark's residual workload at its full size: the 2048x512 O-mesh (1,572,864 cells)
+    in Hilbert order, HLLC + weighted least squares + Venkatakrishnan K=5 at M0.8, 1.25 deg.
+    The library selects the wave-task form by itself; its residual and time steps match the
+    oracle per cell (1e-12 of the local flux scale), repeat bitwise, and equal the quad form's
+    bitwise."""
+    import torch
+    raw = meshgen.mesh_k2l()
+    raw = raw.permuted(fvx.hilbert_permutation(raw))
+    u = states.perturbed_freestream(raw, Minf=0.8, aoa_deg=1.25, L=1.0)
+    phys = dict(Minf=0.8, aoa_deg=1.25, bcs=AIRFOIL_BCS)
+    num = dict(flux="HLLC", gradient="LEASTSQUARES", recon="VENKATAKRISHNAN", limiter_param=5.0)
+    disc = fvx.FlowFV(fvx.Mesh(raw), fvx.physics_cfg(**phys), fvx.numerics_cfg(**num))
+    assert disc.get_launch("flux_wave")[0] == 64
+    ut = torch.tensor(u, device="cuda")
+
+    def evaluate():
+        r = torch.zeros((raw.nelem, 4), dtype=torch.float64, device="cuda")
+        dt = torch.zeros(raw.nelem, dtype=torch.float64, device="cuda")
+        disc.compute_residual(ut, r, True, dt)
+        torch.cuda.synchronize()
+        return r, dt
+
+    r1, dt1 = evaluate()
+    r2, dt2 = evaluate()
+    assert torch.equal(r1, r2) and torch.equal(dt1, dt2)
+    od = po.Disc(po.Mesh(raw), po.make_config(**phys, **num))
+    r_ref, dt_ref = od.residual(u)
+    scale = np.maximum(np.abs(r_ref), od.flux_magnitude(u))
+    rel = np.abs(r1.cpu().numpy() - r_ref) / scale
+    assert rel.max() <= 1e-12, rel.max()
+    assert np.allclose(dt1.cpu().numpy(), dt_ref, rtol=1e-12, atol=0)
+    disc.set_launch("flux", 256, True)
+    assert disc.get_launch("flux_wave")[0] == 0
+    rq, dtq = evaluate()
+    assert torch.equal(r1, rq) and torch.equal(dt1, dtq)
