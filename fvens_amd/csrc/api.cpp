@@ -597,6 +597,8 @@ fvx_status fvx_set_launch(fvx_disc d, int kernel, int block, int xcd)
 		}
 		else if(kernel == FVX_KERNEL_FLUX_WAVE) {
 			require(block == 64, "the wave-task flux kernel runs one wave64 per workgroup");
+			require(fvx::flux_wave_apply(*d), "the wave-task flux form applies to inviscid "
+			        "first-order or linear reconstruction only");
 			d->tune.flux_xcd = xcd; d->tune.flux_wave = 1;
 		}
 		else if(kernel == FVX_KERNEL_JACOBIAN) {

@@ -418,8 +418,9 @@ fvx_status fvx_mpnn_train_step(fvx_mpnn n, int ngraph, const float* targets, con
 /* Kernel ids for fvx_profile_read and fvx_set_launch / fvx_get_launch. FVX_KERNEL_FLUX_WAVE
  * is the wave-task form of the face-flux pass (inviscid, first-order or linear reconstruction):
  * each wave64 owns a group of consecutive cells and evaluates each face touching them once;
- * setting a launch for it (block 64) selects that form, setting one for FVX_KERNEL_FLUX selects
- * the quad-per-cell form. Both agree with the reference to rounding; its time is read as
+ * setting a launch for it (block 64) selects that form (FVX_ERR_INVALID for a viscous or MUSCL
+ * discretisation), setting one for FVX_KERNEL_FLUX selects the quad-per-cell form. The two
+ * forms give bitwise equal residuals and time steps; the wave form's time is read as
  * FVX_KERNEL_FLUX. */
 enum { FVX_KERNEL_GRADIENTS = 0, FVX_KERNEL_FLUX = 1, FVX_KERNEL_JACOBIAN = 2,
        FVX_KERNEL_PC = 3, FVX_KERNEL_FLUX_WAVE = 4 };

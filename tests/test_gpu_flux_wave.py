@@ -142,10 +142,15 @@ def test_the_form_is_selected_by_the_cell_order(torch_cuda):
     d = fvx.FlowFV(fvx.Mesh(r), fvx.physics_cfg(**phys),
                    fvx.numerics_cfg(flux="HLLC", gradient="LEASTSQUARES", recon="VANALBADA"))
     assert d.get_launch("flux_wave")[0] == 0
+    with pytest.raises(fvx.FvxError):
+        d.set_launch("flux_wave", 64, True)
     d = fvx.FlowFV(fvx.Mesh(r), fvx.physics_cfg(Minf=0.5, Tinf=288.15, Reinf=5000.0, Pr=0.72,
                                                  viscous=True,
                                                  bcs=[(2, "adiabaticwall"), (4, "farfield")]),
                    fvx.numerics_cfg(**num))
+    assert d.get_launch("flux_wave")[0] == 0
+    with pytest.raises(fvx.FvxError):
+        d.set_launch("flux_wave", 64, True)
     assert d.get_launch("flux_wave")[0] == 0
 
 
