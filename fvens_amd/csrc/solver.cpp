@@ -428,6 +428,7 @@ struct Krylov {
 	int m = 0, cap = 0;
 	DBuf<double> V, Z, partial, scal, kst;
 	DBuf<int> dlist, dlist_in, dids, dits;  // caller's lists / running list / cycle / counts
+	std::vector<int> dlist_ids;             // the list dlist holds on the device
 	double* hpin = nullptr;
 	int* ipin = nullptr;
 	int* hcnt = nullptr;                    // [m + 2] running instances after each step
@@ -448,6 +449,7 @@ struct Krylov {
 		scal.alloc((size_t)cap*NSCAL);
 		kst.alloc((size_t)cap*fgmres_state_size());
 		dlist.alloc(cap);
+		dlist_ids.clear();
 		dlist_in.alloc(cap);
 		dids.alloc(cap);
 		dits.alloc(cap);
@@ -479,14 +481,19 @@ struct Krylov {
 	double& h(int i, int slot) { return hpin[(size_t)i*NSCAL + slot]; }
 
 	/// Launch description for the instances `ids`: no list for a single-instance solve,
-	/// otherwise the list uploaded to the device (the upload is waited for, so the pinned
-	/// staging buffer can be reused at once).
+	/// otherwise the list on the device. A list that differs from the one the device holds is
+	/// uploaded and the upload waited for, so the pinned staging buffer can be reused at once;
+	/// an unchanged list (most pseudo-steps of a batch) costs neither a copy nor a wait.
 	Inst inst(const std::vector<int>& ids, bool single, hipStream_t s)
 	{
 		if(single) return Inst{nullptr, 1};
-		std::memcpy(ipin, ids.data(), ids.size()*sizeof(int));
-		FVX_HIP(hipMemcpyAsync(dlist.p, ipin, ids.size()*sizeof(int), hipMemcpyHostToDevice, s));
-		FVX_HIP(hipStreamSynchronize(s));
+		if(ids != dlist_ids) {
+			std::memcpy(ipin, ids.data(), ids.size()*sizeof(int));
+			FVX_HIP(hipMemcpyAsync(dlist.p, ipin, ids.size()*sizeof(int), hipMemcpyHostToDevice,
+			                       s));
+			FVX_HIP(hipStreamSynchronize(s));
+			dlist_ids = ids;
+		}
 		return Inst{dlist.p, (int)ids.size()};
 	}
 
