@@ -50,6 +50,8 @@ def test_batched_residual_equals_single_residuals(torch_cuda):
                                 dict(pc="ilu0", pc_sweeps=2, pc_precision="f32",
                                      jac_precision="f32"),
                                 dict(pc="sgs", pc_sweeps=4, matrix_free=True),
+                                dict(pc="sgs", pc_sweeps=4, pc_precision="bf16",
+                                     jac_precision="f32", matrix_free=True),
                                 dict(pc="line", pc_sweeps=2, line_threshold=4.0),
                                 dict(implicit=False, cflinit=0.5, cflfin=0.5)])
 def test_batched_solve_equals_single_solves(torch_cuda, kw):
