@@ -25,12 +25,20 @@ def _run(*args):
     return p
 
 
-@pytest.mark.parametrize("precision", ["f64", "f32"])
+@pytest.mark.parametrize("precision", ["f64", "f32", "bf16"])
 def test_cpp_case_driver_naca0012_regression(torch_cuda, precision):
-    p = _run("--mesh", golden("naca0012luo.msh"), "--precision", precision, "--maxiter", "400",
+    # bfloat16 factors stop the case's own tolerance at another point of the approach, 1.4e-7
+    # off in CDp; run to 1e-10 they reach the regression values within 1e-7
+    extra = ["--tol", "1e-10", "--maxiter", "1000"] if precision == "bf16" else ["--maxiter", "400"]
+    p = _run("--mesh", golden("naca0012luo.msh"), "--precision", precision, *extra,
              "--regression", golden("naca0012_regr_MUSCL_LeastSquares_HLLC.txt"),
              "--regression-tol", "1e-7")
     assert p.returncode == 0 and "REGRESSION PASSED" in p.stdout
+
+
+def test_cpp_case_driver_rejects_an_unknown_precision(torch_cuda):
+    p = _run("--mesh", golden("naca0012luo.msh"), "--precision", "f16", "--maxiter", "2")
+    assert p.returncode == 2
 
 
 def test_cpp_adapter_maps_status_to_reference_exceptions(torch_cuda):
